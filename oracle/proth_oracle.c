@@ -1,0 +1,877 @@
+/*
+ * proth_oracle.c -- TEST INFRASTRUCTURE ONLY (see proth_oracle.h).
+ *
+ * A scalar C restatement of ProthLogic 4.2's CPU Proth path, written from
+ * the reference's algorithm (not its AVX-512 code).  It is the "reference
+ * CPU engine" the HIP product is compared against on seeded inputs, and the
+ * portable CPU baseline when the reference itself cannot be built.
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   or_fft_inplace (Stockham mixed radix-8, Bailey four-step)
+ *       -> brute-force DFT for L<=64 and numpy.fft for larger L; IFFT(FFT(x))
+ *          = L*x.  A wrong twiddle index or sign fails these.
+ *   or_pack / pointwise / or_unpack / or_square
+ *       -> exact x^2 mod N against Python int / GMP on random x < N for
+ *          fitting L; closed-form chunk layout for pack.
+ *   or_gate_rejects      -> brute-force (k*2^n+1) mod p for p <= 67.
+ *   or_choose_base       -> GMP mpz_jacobi over the reference's base order.
+ *   or_seed / or_execute_proth_test
+ *       -> GMP mpz_powm verdicts and Res64 (known primes 3*2^20909+1,
+ *          3*2^34350+1, composites), the transcript KATs of the reference.
+ *   or_auto_tune_fft_len -> the L values the reference transcript prints
+ *          for n=120002 (8192), n=5435752 (524288), n=2705367 (262144).
+ *   or_save/load_checkpoint -> byte layout + save->load round trip.
+ * When the reference sources are present, oracle/_ref (the reference's own
+ * fft_engine.cpp built by oracle/Makefile) is compared limb-for-limb with
+ * or_square.
+ *
+ * Deliberate differences from the reference, each one stated where it is:
+ *   * proth_reduce_stage1 + GMP fallback (P/proth_logic.cpp:250-334,
+ *     505-533) is restated by its end effect, an exact v mod N.
+ *   * mpz_powm for the seed (P/proth_logic.cpp:438-440) is restated as
+ *     binary powering with this engine at a length that fits.
+ *   * mpz_jacobi (P/proth_logic.cpp:428,433) is restated as a Jacobi symbol
+ *     by quadratic reciprocity, since the bases are small.
+ * Everything else -- including the reference's defects (no-fit truncation,
+ * resume off-by-one, silent gate) -- is restated as written.
+ */
+#include "proth_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+typedef unsigned __int128 u128;
+
+struct or_ctx {
+    size_t L;
+    double *re, *im, *re2, *im2;
+    double *w1r, *w1i, *w2r, *w2i, *w3r, *w3i;
+    double *Ar, *Ai, *Br, *Bi, *ABr, *ABi;
+    double last_max_diff;
+    void* mem;
+};
+
+/* ---------------------------------------------------------------- init */
+
+/* P/fft_engine.cpp:18-73.  One block of 18 tables of (L+64) doubles; the
+ * twiddles are w_p[i] = exp(-2 pi i * p * i / L) for p = 1, 2, 3.  The
+ * unused DWT weights (:66-67) and the vestigial proth_state_N*.bin load
+ * (:69-71) are omitted. */
+or_ctx* or_fft_init(size_t L)
+{
+    or_ctx* c = (or_ctx*)calloc(1, sizeof(or_ctx));
+    if (!c) return NULL;
+    size_t tab = L + 64;
+    double* m = (double*)aligned_alloc(64, ((tab * 18 * sizeof(double)) + 63) / 64 * 64);
+    if (!m) { free(c); return NULL; }
+    memset(m, 0, tab * 18 * sizeof(double));
+    c->mem = m;
+    c->L = L;
+    double** slots[18] = { &c->re, &c->im, &c->re2, &c->im2, &c->w1r, &c->w1i, &c->w2r, &c->w2i,
+                           &c->w3r, &c->w3i, NULL, NULL, &c->Ar, &c->Ai, &c->Br, &c->Bi, &c->ABr, &c->ABi };
+    for (int s = 0; s < 18; ++s)
+        if (slots[s]) *slots[s] = m + (size_t)s * tab;
+    for (size_t i = 0; i < L; ++i) {
+        double ang = -2.0 * M_PI * (double)i / (double)L;
+        c->w1r[i] = cos(ang);       c->w1i[i] = sin(ang);
+        c->w2r[i] = cos(2.0 * ang); c->w2i[i] = sin(2.0 * ang);
+        c->w3r[i] = cos(3.0 * ang); c->w3i[i] = sin(3.0 * ang);
+    }
+    c->last_max_diff = 0.0;
+    return c;
+}
+
+void or_fft_destroy(or_ctx* c)
+{
+    if (!c) return;
+    free(c->mem);
+    free(c);
+}
+
+size_t or_fft_len(const or_ctx* c) { return c->L; }
+double or_last_max_diff(const or_ctx* c) { return c->last_max_diff; }
+
+/* ------------------------------------------------------------ butterflies */
+
+/* Small DFTs with the reference's sign convention: sign=+1 forward
+ * (multiply by -i for the quarter turn), sign=-1 inverse. */
+static inline void dft2(double* xr, double* xi)
+{
+    double ar = xr[0] + xr[1], ai = xi[0] + xi[1];
+    double br = xr[0] - xr[1], bi = xi[0] - xi[1];
+    xr[0] = ar; xi[0] = ai; xr[1] = br; xi[1] = bi;
+}
+
+static inline void dft4(double* xr, double* xi, double sign)
+{
+    double s0r = xr[0] + xr[2], s0i = xi[0] + xi[2];
+    double s1r = xr[0] - xr[2], s1i = xi[0] - xi[2];
+    double s2r = xr[1] + xr[3], s2i = xi[1] + xi[3];
+    double s3r = xr[1] - xr[3], s3i = xi[1] - xi[3];
+    /* -i*sign*s3 */
+    double dr = s3i * sign, di = -s3r * sign;
+    xr[0] = s0r + s2r; xi[0] = s0i + s2i;
+    xr[1] = s1r + dr;  xi[1] = s1i + di;
+    xr[2] = s0r - s2r; xi[2] = s0i - s2i;
+    xr[3] = s1r - dr;  xi[3] = s1i - di;
+}
+
+/* DFT-8 as two DFT-4s over even/odd inputs joined by the eighth roots. */
+static inline void dft8(double* xr, double* xi, double sign)
+{
+    const double h = 0.70710678118654752440;
+    double er[4] = { xr[0], xr[2], xr[4], xr[6] }, ei[4] = { xi[0], xi[2], xi[4], xi[6] };
+    double orr[4] = { xr[1], xr[3], xr[5], xr[7] }, oi[4] = { xi[1], xi[3], xi[5], xi[7] };
+    dft4(er, ei, sign);
+    dft4(orr, oi, sign);
+    /* W8^q = exp(-i*sign*pi*q/4) */
+    double wr[4] = { 1.0, h, 0.0, -h };
+    double wi[4] = { 0.0, -h * sign, -sign, -h * sign };
+    for (int q = 0; q < 4; ++q) {
+        double tr = orr[q] * wr[q] - oi[q] * wi[q];
+        double ti = orr[q] * wi[q] + oi[q] * wr[q];
+        xr[q] = er[q] + tr;     xi[q] = ei[q] + ti;
+        xr[q + 4] = er[q] - tr; xi[q + 4] = ei[q] - ti;
+    }
+}
+
+/* ------------------------------------------------ Stockham mixed radix-8 */
+
+/* P/fft_engine.cpp:234-494.  Passes of radix 8 while n/m >= 8, then one of
+ * radix 4 or 2.  Pass with span m: inputs k + j*r*stride + p*stride,
+ * twiddle w1[p*j*stride] (conjugated for the inverse), outputs
+ * k + j*stride + q*m*stride.  Ping-pongs between (in, out); returns the
+ * buffer holding the result. */
+static double* stockham_mixed(const or_ctx* c, double* in_re, double* in_im,
+                              double* out_re, double* out_im, int inverse, double** res_im)
+{
+    const size_t n = c->L;
+    const double sign = inverse ? -1.0 : 1.0;
+    double xr[8], xi[8];
+    for (size_t m = 1; m < n;) {
+        size_t remaining = n / m;
+        size_t r = remaining >= 8 ? 8 : (remaining >= 4 ? 4 : 2);
+        size_t stride = remaining / r;
+        for (size_t j = 0; j < m; ++j) {
+            size_t tw = j * stride;
+            double wr[8], wi[8];
+            for (size_t p = 1; p < r; ++p) {
+                wr[p] = c->w1r[p * tw];
+                wi[p] = inverse ? -c->w1i[p * tw] : c->w1i[p * tw];
+            }
+            for (size_t k = 0; k < stride; ++k) {
+                size_t i0 = k + j * r * stride;
+                xr[0] = in_re[i0]; xi[0] = in_im[i0];
+                for (size_t p = 1; p < r; ++p) {
+                    double ar = in_re[i0 + p * stride], ai = in_im[i0 + p * stride];
+                    xr[p] = ar * wr[p] - ai * wi[p];
+                    xi[p] = ar * wi[p] + ai * wr[p];
+                }
+                if (r == 8) dft8(xr, xi, sign);
+                else if (r == 4) dft4(xr, xi, sign);
+                else dft2(xr, xi);
+                size_t o0 = k + j * stride;
+                for (size_t q = 0; q < r; ++q) {
+                    out_re[o0 + q * m * stride] = xr[q];
+                    out_im[o0 + q * m * stride] = xi[q];
+                }
+            }
+        }
+        m *= r;
+        double* t;
+        t = in_re; in_re = out_re; out_re = t;
+        t = in_im; in_im = out_im; out_im = t;
+    }
+    *res_im = in_im;
+    return in_re;
+}
+
+/* --------------------------------------------------- Bailey four-step */
+
+/* P/fft_engine.cpp:81-103: dst[jj*n1+ii] = src[ii*n2+jj]. */
+static void transpose(const double* sr, const double* si, double* dr, double* di, size_t n1, size_t n2)
+{
+    for (size_t ii = 0; ii < n1; ++ii)
+        for (size_t jj = 0; jj < n2; ++jj) {
+            dr[jj * n1 + ii] = sr[ii * n2 + jj];
+            di[jj * n1 + ii] = si[ii * n2 + jj];
+        }
+}
+
+/* P/fft_engine.cpp:105-136: dst[jj*n2+ii] = src[ii*n1+jj] * w1[jj*ii]
+ * (conjugate twiddle for the inverse). */
+static void transpose_twiddle(const or_ctx* c, const double* sr, const double* si, double* dr, double* di,
+                              size_t n2, size_t n1, int inverse)
+{
+    for (size_t ii = 0; ii < n2; ++ii)
+        for (size_t jj = 0; jj < n1; ++jj) {
+            double r = sr[ii * n1 + jj], im = si[ii * n1 + jj];
+            size_t idx = jj * ii;
+            double wr = c->w1r[idx], wi = inverse ? -c->w1i[idx] : c->w1i[idx];
+            dr[jj * n2 + ii] = r * wr - im * wi;
+            di[jj * n2 + ii] = r * wi + im * wr;
+        }
+}
+
+/* P/fft_engine.cpp:499-698 (all vector branches compute the same pass):
+ * radix-4 Stockham over a row of length n (a power of 4) with twiddles
+ * W_n^{p*j*stride} taken from the global tables at stride L/n, exactly the
+ * per-row vectors built at :715-722 / :740-747.  Result in `in` (copied
+ * back as at :729-732). */
+static void row_fft_radix4(const or_ctx* c, double* in_re, double* in_im, double* out_re, double* out_im,
+                           size_t n, int inverse)
+{
+    const double sign = inverse ? -1.0 : 1.0;
+    const size_t tws = c->L / n;
+    double* a_re = in_re; double* a_im = in_im;
+    double* b_re = out_re; double* b_im = out_im;
+    double xr[4], xi[4];
+    for (size_t m = 1; m < n; m <<= 2) {
+        size_t stride = n / (m << 2);
+        for (size_t j = 0; j < m; ++j) {
+            size_t tw = j * stride * tws;
+            double w1r = c->w1r[tw], w1i = inverse ? -c->w1i[tw] : c->w1i[tw];
+            double w2r = c->w2r[tw], w2i = inverse ? -c->w2i[tw] : c->w2i[tw];
+            double w3r = c->w3r[tw], w3i = inverse ? -c->w3i[tw] : c->w3i[tw];
+            for (size_t k = 0; k < stride; ++k) {
+                size_t i0 = k + j * 4 * stride;
+                double r1 = a_re[i0 + stride], i1 = a_im[i0 + stride];
+                double r2 = a_re[i0 + 2 * stride], i2 = a_im[i0 + 2 * stride];
+                double r3 = a_re[i0 + 3 * stride], i3 = a_im[i0 + 3 * stride];
+                xr[0] = a_re[i0]; xi[0] = a_im[i0];
+                xr[1] = r1 * w1r - i1 * w1i; xi[1] = r1 * w1i + i1 * w1r;
+                xr[2] = r2 * w2r - i2 * w2i; xi[2] = r2 * w2i + i2 * w2r;
+                xr[3] = r3 * w3r - i3 * w3i; xi[3] = r3 * w3i + i3 * w3r;
+                dft4(xr, xi, sign);
+                size_t o0 = k + j * stride;
+                for (int q = 0; q < 4; ++q) {
+                    b_re[o0 + q * m * stride] = xr[q];
+                    b_im[o0 + q * m * stride] = xi[q];
+                }
+            }
+        }
+        double* t;
+        t = a_re; a_re = b_re; b_re = t;
+        t = a_im; a_im = b_im; b_im = t;
+    }
+    if (a_re != in_re) {
+        memcpy(in_re, a_re, n * sizeof(double));
+        memcpy(in_im, a_im, n * sizeof(double));
+    }
+}
+
+/* P/fft_engine.cpp:700-820.  L = n1*n2 with n1, n2 powers of 4 and
+ * n1 >= n2 (:702-707).  Operates on (re, im) with (re2, im2) as scratch;
+ * result left in (re, im). */
+static void bailey_2d(or_ctx* c, double* re, double* im, double* re2, double* im2, int inverse)
+{
+    size_t N = c->L, n1 = 1, n2 = 1, t = N;
+    while (t > 1) {
+        if (n1 <= n2) n1 *= 4; else n2 *= 4;
+        t /= 4;
+    }
+    if (!inverse) {
+        transpose(re, im, re2, im2, n1, n2);
+        for (size_t i = 0; i < n2; ++i) row_fft_radix4(c, re2 + i * n1, im2 + i * n1, re + i * n1, im + i * n1, n1, 0);
+        transpose_twiddle(c, re2, im2, re, im, n2, n1, 0);
+        for (size_t i = 0; i < n1; ++i) row_fft_radix4(c, re + i * n2, im + i * n2, re2 + i * n2, im2 + i * n2, n2, 0);
+        transpose(re, im, re2, im2, n1, n2);
+    } else {
+        transpose(re, im, re2, im2, n2, n1);
+        for (size_t i = 0; i < n1; ++i) row_fft_radix4(c, re2 + i * n2, im2 + i * n2, re + i * n2, im + i * n2, n2, 1);
+        transpose_twiddle(c, re2, im2, re, im, n1, n2, 1);
+        for (size_t i = 0; i < n2; ++i) row_fft_radix4(c, re + i * n1, im + i * n1, re2 + i * n1, im2 + i * n1, n1, 1);
+        transpose(re, im, re2, im2, n2, n1);
+    }
+    memcpy(re, re2, N * sizeof(double));
+    memcpy(im, im2, N * sizeof(double));
+}
+
+static unsigned log2_exact(size_t L)
+{
+    unsigned p = 0;
+    while (L > 1) { L >>= 1; ++p; }
+    return p;
+}
+
+/* P/fft_engine.cpp:994-1014: even log2 L -> Bailey, odd -> Stockham. */
+void or_fft_inplace(or_ctx* c, double* re, double* im, int inverse)
+{
+    unsigned p = log2_exact(c->L);
+    if (p % 2 == 0) {
+        bailey_2d(c, re, im, c->re2, c->im2, inverse);
+    } else {
+        double* res_im;
+        double* res_re = stockham_mixed(c, re, im, c->re2, c->im2, inverse, &res_im);
+        if (res_re != re) {
+            memcpy(re, res_re, c->L * sizeof(double));
+            memcpy(im, res_im, c->L * sizeof(double));
+        }
+    }
+}
+
+/* ------------------------------------------------------------- pack */
+
+/* P/llr_fft_pack.cpp:12-41.  A_re[j] = chunk j, B_re[j] = chunk j+L/2,
+ * j < L/2, 16-bit chunks; everything else zero.  Chunks are read in groups
+ * of 8 when the group start is below 4*n_limbs, as the vector loads do. */
+void or_pack(or_ctx* c, const uint64_t* in, size_t n_limbs)
+{
+    const size_t L = c->L, half = L / 2;
+    memset(c->Ar, 0, L * sizeof(double));
+    memset(c->Ai, 0, L * sizeof(double));
+    memset(c->Br, 0, L * sizeof(double));
+    memset(c->Bi, 0, L * sizeof(double));
+    const uint16_t* in16 = (const uint16_t*)in;
+    size_t total = n_limbs * 4;
+    for (size_t j = 0; j < half; j += 8) {
+        for (size_t t = 0; t < 8 && j + t < half; ++t) {
+            c->Ar[j + t] = (j < total) ? (double)in16[j + t] : 0.0;
+            size_t jb = j + half;
+            c->Br[j + t] = (jb < total) ? (double)in16[jb + t] : 0.0;
+        }
+    }
+}
+
+/* ------------------------------------------------------------ unpack */
+
+/* P/fft_engine.cpp:825-984. */
+void or_unpack(or_ctx* c, uint64_t* out, size_t n_limbs, unsigned n_bits, uint64_t k_val)
+{
+    const size_t L = c->L, half = L / 2;
+    const size_t max_chunks = n_limbs * 8;
+    const size_t dl = n_limbs * 2 + 2;
+    uint64_t* sq = (uint64_t*)calloc(dl, sizeof(uint64_t));
+    int64_t carry = 0;
+    double max_err = 0.0;
+
+    /* :836-870 overlap-add of A^2, 2AB (shifted L/2), B^2 (shifted L);
+     * round to nearest, track |s - rint(s)|, signed carry in base 2^16. */
+    for (size_t i = 0; i < max_chunks; i += 8) {
+        uint64_t acc0 = 0, acc1 = 0;
+        for (size_t t = 0; t < 8; ++t) {
+            size_t ii = i + t;
+            double a2 = (i < L) ? c->Ar[ii] : 0.0;
+            double ab = ((i >= half) && ((i - half) < L)) ? c->ABr[ii - half] : 0.0;
+            double b2 = ((i >= 2 * half) && ((i - 2 * half) < L)) ? c->Br[ii - 2 * half] : 0.0;
+            double s = (a2 + ab) + b2;
+            int64_t iv = llrint(s);
+            double e = fabs(s - (double)iv);
+            if (e > max_err) max_err = e;
+            int64_t v = iv + carry;
+            carry = v >> 16;
+            uint64_t chunk = (uint64_t)(v & 0xFFFF);
+            if (t < 4) acc0 |= chunk << (t * 16);
+            else acc1 |= chunk << ((t - 4) * 16);
+        }
+        size_t lb = i >> 2;
+        if (lb < dl) sq[lb] |= acc0;
+        if (lb + 1 < dl) sq[lb + 1] |= acc1;
+    }
+    c->last_max_diff = max_err;
+
+    /* :880-886 final positive carry. */
+    size_t ci = max_chunks >> 2;
+    while (carry > 0 && ci < dl) {
+        u128 s = (u128)sq[ci] + (uint64_t)carry;
+        sq[ci] = (uint64_t)s;
+        carry = (int64_t)(s >> 64);
+        ++ci;
+    }
+
+    /* :888-904 H = sq >> n over n_limbs limbs. */
+    size_t split_q = n_bits / 64;
+    unsigned split_r = n_bits % 64;
+    uint64_t* H = (uint64_t*)calloc(n_limbs + 2, sizeof(uint64_t));
+    uint64_t* Q = (uint64_t*)calloc(n_limbs + 2, sizeof(uint64_t));
+    for (size_t i = 0; i < n_limbs; ++i) {
+        size_t s = split_q + i;
+        uint64_t h = 0;
+        if (s < dl) {
+            h = sq[s] >> split_r;
+            if (split_r > 0 && s + 1 < dl) h |= sq[s + 1] << (64 - split_r);
+        }
+        H[i] = h;
+    }
+
+    /* :909-923 Q = H / k, rem = H mod k (serial long division). */
+    uint64_t rem = 0;
+    for (size_t ii = n_limbs; ii-- > 0;) {
+        u128 num = ((u128)rem << 64) | H[ii];
+        Q[ii] = (uint64_t)(num / k_val);
+        rem = (uint64_t)(num % k_val);
+    }
+
+    /* :925-931 low n bits. */
+    memset(out, 0, n_limbs * sizeof(uint64_t));
+    for (size_t i = 0; i < split_q && i < n_limbs; ++i) out[i] = sq[i];
+    if (split_r > 0 && split_q < n_limbs) out[split_q] = sq[split_q] & ((1ULL << split_r) - 1);
+
+    /* :933-954 + rem * 2^n (dropped when split_q >= n_limbs: no-fit). */
+    if (rem > 0 && split_q < n_limbs) {
+        uint64_t rem_L = (split_r == 0) ? rem : (rem << split_r);
+        uint64_t rem_H = (split_r == 0) ? 0 : (rem >> (64 - split_r));
+        u128 s = (u128)out[split_q] + rem_L;
+        out[split_q] = (uint64_t)s;
+        uint64_t cadd = (uint64_t)(s >> 64);
+        if (split_q + 1 < n_limbs) {
+            s = (u128)out[split_q + 1] + rem_H + cadd;
+            out[split_q + 1] = (uint64_t)s;
+            cadd = (uint64_t)(s >> 64);
+            size_t idx = split_q + 2;
+            while (cadd > 0 && idx < n_limbs) {
+                s = (u128)out[idx] + cadd;
+                out[idx] = (uint64_t)s;
+                cadd = (uint64_t)(s >> 64);
+                ++idx;
+            }
+        }
+    }
+
+    /* :959-964 - Q. */
+    unsigned borrow = 0;
+    for (size_t i = 0; i < n_limbs; ++i) {
+        uint64_t a = out[i], b = Q[i];
+        uint64_t d = a - b - borrow;
+        borrow = (a < b) || (a - b < borrow);
+        out[i] = d;
+    }
+    /* :966-983 + N on borrow. */
+    if (borrow) {
+        uint64_t lc = 1;
+        for (size_t i = 0; i < n_limbs; ++i) {
+            uint64_t v = lc;
+            lc = 0;
+            if (i == split_q) v += (split_r == 0) ? k_val : (k_val << split_r);
+            if (i == split_q + 1 && split_r > 0) v += (k_val >> (64 - split_r));
+            u128 s = (u128)out[i] + v;
+            out[i] = (uint64_t)s;
+            lc += (uint64_t)(s >> 64);
+        }
+    }
+    free(sq);
+    free(H);
+    free(Q);
+}
+
+/* ------------------------------------------------------------ square */
+
+/* P/fft_engine.cpp:986-1073: pack, 2 forward FFTs, pointwise
+ * (A^2/L, B^2/L, 2AB/L from the untouched transforms, :1023-1056),
+ * 3 inverse FFTs, unpack. */
+void or_square(or_ctx* c, uint64_t* v, size_t n_limbs, unsigned n_bits, uint64_t k)
+{
+    const size_t L = c->L;
+    or_pack(c, v, n_limbs);
+    or_fft_inplace(c, c->Ar, c->Ai, 0);
+    or_fft_inplace(c, c->Br, c->Bi, 0);
+    const double inv_n = 1.0 / (double)L;
+    for (size_t i = 0; i < L; ++i) {
+        double ar = c->Ar[i], ai = c->Ai[i], br = c->Br[i], bi = c->Bi[i];
+        c->Ar[i] = (ar * ar - ai * ai) * inv_n;
+        c->Ai[i] = (2.0 * (ar * ai)) * inv_n;
+        c->Br[i] = (br * br - bi * bi) * inv_n;
+        c->Bi[i] = (2.0 * (br * bi)) * inv_n;
+        c->ABr[i] = ((ar * br - ai * bi) * 2.0) * inv_n;
+        c->ABi[i] = ((ar * bi + ai * br) * 2.0) * inv_n;
+    }
+    or_fft_inplace(c, c->Ar, c->Ai, 1);
+    or_fft_inplace(c, c->Br, c->Bi, 1);
+    or_fft_inplace(c, c->ABr, c->ABi, 1);
+    or_unpack(c, v, n_limbs, n_bits, k);
+}
+
+/* ----------------------------------------------------- scalar number theory */
+
+static uint64_t mulmod(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)(((u128)a * b) % m); }
+
+static uint64_t powmod(uint64_t a, uint64_t e, uint64_t m)
+{
+    uint64_t r = 1 % m;
+    a %= m;
+    while (e) {
+        if (e & 1) r = mulmod(r, a, m);
+        a = mulmod(a, a, m);
+        e >>= 1;
+    }
+    return r;
+}
+
+/* P/proth_logic.cpp:41-50: p | k*2^n+1  <=>  2^n == -(k^-1) (mod p). */
+static int gate_prime(uint64_t k, unsigned n, uint64_t p)
+{
+    if (p % 2 == 0) return 0;
+    uint64_t km = k % p;
+    if (km == 0) return 0;
+    uint64_t invk = powmod(km, p - 2, p);
+    uint64_t target = mulmod(p - 1, invk, p);
+    return powmod(2, n, p) == target;
+}
+
+/* P/proth_logic.cpp:405-408. */
+int or_gate_rejects(uint64_t k, unsigned n)
+{
+    static const uint64_t ps[] = { 2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67 };
+    for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); ++i)
+        if (gate_prime(k, n, ps[i])) return 1;
+    return 0;
+}
+
+/* Jacobi symbol (a/m), m odd positive (textbook binary algorithm). */
+static int jacobi_u64(uint64_t a, uint64_t m)
+{
+    int t = 1;
+    a %= m;
+    while (a != 0) {
+        while ((a & 1) == 0) {
+            a >>= 1;
+            uint64_t r = m & 7;
+            if (r == 3 || r == 5) t = -t;
+        }
+        uint64_t tmp = a; a = m; m = tmp;
+        if ((a & 3) == 3 && (m & 3) == 3) t = -t;
+        a %= m;
+    }
+    return (m == 1) ? t : 0;
+}
+
+/* (b / N) for N = k*2^n+1 and small b, by quadratic reciprocity:
+ * strip factors of 2 using N mod 8, then (b'/N) = (N mod b' / b') times
+ * (-1)^{((b'-1)/2)((N-1)/2)}. */
+int or_jacobi_small(uint64_t b, uint64_t k, unsigned n)
+{
+    /* N mod 8 */
+    uint64_t n8 = ((k % 8) * powmod(2, n, 8) + 1) % 8;
+    int t = 1;
+    while (b && (b & 1) == 0) {
+        b >>= 1;
+        if (n8 == 3 || n8 == 5) t = -t;
+    }
+    if (b == 0) return 0;
+    if (b == 1) return t;
+    /* (N-1)/2 = k*2^(n-1): its parity */
+    uint64_t nm1_half_odd = (n == 0) ? 0 : ((n == 1) ? (k & 1) : 0);
+    if (n == 0) nm1_half_odd = 0; /* N = k+1: handled generically below */
+    uint64_t Nmod = ((k % b) * powmod(2, n, b) + 1) % b;
+    if (n == 0) {
+        /* N = k+1, use the direct symbol */
+        return t * jacobi_u64(b, k + 1);
+    }
+    if (((b - 1) / 2) % 2 == 1 && nm1_half_odd) t = -t;
+    if (Nmod == 0) return 0;
+    return t * jacobi_u64(Nmod, b);
+}
+
+/* P/proth_logic.cpp:423-435. */
+uint64_t or_choose_base(uint64_t k, unsigned n)
+{
+    static const uint64_t pref[] = { 2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37 };
+    for (size_t i = 0; i < sizeof(pref) / sizeof(pref[0]); ++i)
+        if (or_jacobi_small(pref[i], k, n) == -1) return pref[i];
+    for (uint64_t b = 2; b < 100; ++b)
+        if (or_jacobi_small(b, k, n) == -1) return b;
+    return 0;
+}
+
+/* ------------------------------------------------ exact limb arithmetic */
+
+static size_t bitlen_u64(uint64_t x)
+{
+    size_t b = 0;
+    while (x) { ++b; x >>= 1; }
+    return b;
+}
+
+int or_fits(size_t L, unsigned n, uint64_t k)
+{
+    return 16.0 * (double)L >= (double)n + 2.0 * (double)bitlen_u64(k) + 16.0;
+}
+
+/* N as limbs into dst[0..nl) (zero-extended). */
+static void make_N(uint64_t* dst, size_t nl, unsigned n, uint64_t k)
+{
+    memset(dst, 0, nl * sizeof(uint64_t));
+    size_t q = n / 64;
+    unsigned r = n % 64;
+    dst[0] = 1;
+    if (q < nl) dst[q] += (r == 0) ? k : (k << r);
+    if (r && q + 1 < nl) dst[q + 1] += k >> (64 - r);
+}
+
+static int cmp_limbs(const uint64_t* a, const uint64_t* b, size_t nl)
+{
+    for (size_t i = nl; i-- > 0;) {
+        if (a[i] > b[i]) return 1;
+        if (a[i] < b[i]) return -1;
+    }
+    return 0;
+}
+
+static void sub_limbs(uint64_t* a, const uint64_t* b, size_t nl) /* a -= b */
+{
+    unsigned borrow = 0;
+    for (size_t i = 0; i < nl; ++i) {
+        uint64_t x = a[i], y = b[i];
+        uint64_t d = x - y - borrow;
+        borrow = (x < y) || (x - y < borrow);
+        a[i] = d;
+    }
+}
+
+static void add_limbs(uint64_t* a, const uint64_t* b, size_t nl) /* a += b */
+{
+    uint64_t c = 0;
+    for (size_t i = 0; i < nl; ++i) {
+        u128 s = (u128)a[i] + b[i] + c;
+        a[i] = (uint64_t)s;
+        c = (uint64_t)(s >> 64);
+    }
+}
+
+/* v (nl limbs) -> v mod N, exact: v = H*2^n + Lo with H = q*k + r gives
+ * v == Lo + r*2^n - q (mod N) (k*2^n == -1); q is reduced recursively. */
+void or_mod_N(uint64_t* v, size_t nl, unsigned n, uint64_t k)
+{
+    size_t wl = nl + 2;
+    uint64_t* Nl = (uint64_t*)calloc(wl, sizeof(uint64_t));
+    make_N(Nl, wl, n, k);
+    uint64_t* w = (uint64_t*)calloc(wl, sizeof(uint64_t));
+    memcpy(w, v, nl * sizeof(uint64_t));
+    if (cmp_limbs(w, Nl, wl) >= 0) {
+        size_t q = n / 64;
+        unsigned r = n % 64;
+        uint64_t* H = (uint64_t*)calloc(wl, sizeof(uint64_t));
+        for (size_t i = 0; i + q < wl; ++i) {
+            uint64_t h = w[i + q] >> r;
+            if (r && i + q + 1 < wl) h |= w[i + q + 1] << (64 - r);
+            H[i] = h;
+        }
+        uint64_t rem = 0;
+        for (size_t i = wl; i-- > 0;) {
+            u128 num = ((u128)rem << 64) | H[i];
+            H[i] = (uint64_t)(num / k);
+            rem = (uint64_t)(num % k);
+        }
+        /* Lo + rem*2^n */
+        uint64_t* A = (uint64_t*)calloc(wl, sizeof(uint64_t));
+        for (size_t i = 0; i < q && i < wl; ++i) A[i] = w[i];
+        if (r && q < wl) A[q] = w[q] & ((1ULL << r) - 1);
+        uint64_t* R = (uint64_t*)calloc(wl, sizeof(uint64_t));
+        if (q < wl) R[q] = (r == 0) ? rem : (rem << r);
+        if (r && q + 1 < wl) R[q + 1] = rem >> (64 - r);
+        add_limbs(A, R, wl);               /* A < N */
+        or_mod_N(H, wl, n, k);             /* q mod N */
+        if (cmp_limbs(A, H, wl) >= 0) {
+            sub_limbs(A, H, wl);
+        } else {
+            add_limbs(A, Nl, wl);
+            sub_limbs(A, H, wl);
+        }
+        while (cmp_limbs(A, Nl, wl) >= 0) sub_limbs(A, Nl, wl);
+        memcpy(w, A, wl * sizeof(uint64_t));
+        free(H); free(A); free(R);
+    }
+    memcpy(v, w, nl * sizeof(uint64_t));
+    free(w);
+    free(Nl);
+}
+
+/* v <- v*a mod N (v has nl limbs, nl large enough for N). */
+static void mul_small_mod_N(uint64_t* v, size_t nl, uint64_t a, unsigned n, uint64_t k)
+{
+    uint64_t* w = (uint64_t*)calloc(nl + 2, sizeof(uint64_t));
+    uint64_t c = 0;
+    for (size_t i = 0; i < nl; ++i) {
+        u128 p = (u128)v[i] * a + c;
+        w[i] = (uint64_t)p;
+        c = (uint64_t)(p >> 64);
+    }
+    w[nl] = c;
+    or_mod_N(w, nl + 2, n, k);
+    memcpy(v, w, nl * sizeof(uint64_t));
+    free(w);
+}
+
+/* Stands in for mpz_powm(a, k, N) (P/proth_logic.cpp:438-440): left-to-right
+ * binary powering with or_square at a length that fits (or_fits), so the
+ * seed is exact even when c's own length is in the no-fit band. */
+void or_seed(or_ctx* c, uint64_t* limbs, uint64_t a, uint64_t k, unsigned n)
+{
+    size_t L = c->L;
+    or_ctx* w = c;
+    if (!or_fits(L, n, k)) {
+        while (!or_fits(L, n, k)) L *= 2;
+        w = or_fft_init(L);
+    }
+    size_t nl = L / 4;
+    uint64_t* x = (uint64_t*)calloc(nl + 8, sizeof(uint64_t));
+    size_t out_nl = (n + 64 + 63) / 64 + 1;
+    if (a == 0) {
+        memset(limbs, 0, out_nl * sizeof(uint64_t));
+        if (w != c) or_fft_destroy(w);
+        free(x);
+        return;
+    }
+    x[0] = a;
+    or_mod_N(x, nl, n, k);
+    int top = 63;
+    while (top > 0 && !((k >> top) & 1)) --top;
+    for (int b = top - 1; b >= 0; --b) {
+        or_square(w, x, nl, n, k);
+        if ((k >> b) & 1) mul_small_mod_N(x, nl, a, n, k);
+    }
+    size_t cp = out_nl < nl ? out_nl : nl;
+    memcpy(limbs, x, cp * sizeof(uint64_t));
+    for (size_t i = cp; i < out_nl; ++i) limbs[i] = 0;
+    if (w != c) or_fft_destroy(w);
+    free(x);
+}
+
+/* P/proth_logic.cpp:336-355. */
+int or_is_nm1(const uint64_t* limbs, size_t n_limbs, unsigned n, uint64_t k)
+{
+    size_t limb = n >> 6;
+    unsigned shift = n & 63u;
+    for (size_t i = 0; i < n_limbs; ++i) {
+        uint64_t t = 0;
+        if (i == limb) t = k << shift;
+        if (shift != 0 && limb + 1 < n_limbs && i == limb + 1) t = (uint64_t)(((u128)k << shift) >> 64);
+        if (limbs[i] != t) return 0;
+    }
+    return 1;
+}
+
+/* -------------------------------------------------------- checkpoint */
+
+struct or_ckpt_header { uint64_t current_i; size_t n_limbs; };
+
+/* P/checkpoint_manager.cpp:8-23: write .tmp, fflush, fsync, rename. */
+int or_save_checkpoint(uint64_t i, size_t n_limbs, const uint64_t* data, const char* filename)
+{
+    char tmp[4096];
+    snprintf(tmp, sizeof tmp, "%s.tmp", filename);
+    FILE* f = fopen(tmp, "wb");
+    if (!f) return 0;
+    struct or_ckpt_header h = { i, n_limbs };
+    fwrite(&h, sizeof h, 1, f);
+    fwrite(data, sizeof(uint64_t), n_limbs, f);
+    fflush(f);
+    fsync(fileno(f));
+    fclose(f);
+    rename(tmp, filename);
+    return 1;
+}
+
+/* P/checkpoint_manager.cpp:25-45: a header whose n_limbs differs is a
+ * silent "no checkpoint". */
+int or_load_checkpoint(uint64_t* out_i, size_t n_limbs, uint64_t* data, const char* filename)
+{
+    FILE* f = fopen(filename, "rb");
+    if (!f) return 0;
+    struct or_ckpt_header h;
+    if (fread(&h, sizeof h, 1, f) != 1) { fclose(f); return 0; }
+    if (h.n_limbs != n_limbs) { fclose(f); return 0; }
+    if (fread(data, sizeof(uint64_t), n_limbs, f) != n_limbs) { fclose(f); return 0; }
+    *out_i = h.current_i;
+    fclose(f);
+    return 1;
+}
+
+/* ---------------------------------------------------------- autotune */
+
+/* P/dcore.cpp:20-81. */
+size_t or_auto_tune_fft_len(unsigned n, uint64_t k, int fit_fix, double* err_out)
+{
+    size_t n_limbs = (n + 63) / 64;
+    size_t cand = 256;
+    double last_err = -1.0;
+    while ((double)n / (double)cand > 18.0) cand *= 2;
+    for (;;) {
+        double bpw = (double)n / (double)cand;
+        if (bpw < 11.0) break;
+        or_ctx* t = or_fft_init(cand);
+        size_t alloc = cand > (2 * (size_t)n) / 64 + 64 ? cand : (2 * (size_t)n) / 64 + 64;
+        uint64_t* tl = (uint64_t*)calloc(alloc + 8, sizeof(uint64_t));
+        for (size_t i = 0; i < n_limbs; ++i) tl[i] = 0xFFFFFFFFFFFFFFFFULL;
+        tl[0] = k | 1;
+        for (int s = 0; s < 100; ++s) or_square(t, tl, n_limbs, n, k);
+        last_err = or_last_max_diff(t);
+        free(tl);
+        or_fft_destroy(t);
+        if (last_err >= 0.0 && last_err <= 0.30) break;
+        cand *= 2;
+    }
+    if (fit_fix)
+        while (!or_fits(cand, n, k)) cand *= 2;
+    if (err_out) *err_out = last_err;
+    return cand;
+}
+
+/* ----------------------------------------------------------- driver */
+
+static size_t highest_nonzero(const uint64_t* l, size_t nl)
+{
+    for (size_t i = nl; i-- > 0;)
+        if (l[i]) return i;
+    return 0;
+}
+
+/* P/proth_logic.cpp:384-580. */
+int or_execute_proth_test(uint64_t k, unsigned n, uint64_t* limbs, or_ctx* c, const char* ckdir,
+                          uint64_t stop_after, uint64_t* base_out, double* max_err_out, uint64_t* iters_out)
+{
+    if (base_out) *base_out = 0;
+    if (max_err_out) *max_err_out = 0.0;
+    if (iters_out) *iters_out = 0;
+    /* :405-408 silent gate */
+    if (or_gate_rejects(k, n)) return 0;
+    /* :418-435 base */
+    uint64_t a = or_choose_base(k, n);
+    if (base_out) *base_out = a;
+    /* :438-446 seed a^k mod N, exported over at least L/4 limbs */
+    const size_t nls = c->L / 4;
+    memset(limbs, 0, nls * sizeof(uint64_t));
+    or_seed(c, limbs, a, k, n);
+    /* :449-451 silent checkpoint load */
+    char path[4096];
+    uint64_t rec_i = 0;
+    if (ckdir) {
+        snprintf(path, sizeof path, "%s/check_k%llu_n%u.bin", ckdir, (unsigned long long)k, n);
+        or_load_checkpoint(&rec_i, nls, limbs, path);
+    }
+    double mx = 0.0;
+    size_t last_hi = (n + 64) / 64 + 10;
+    uint64_t iters = 0;
+    /* :485-549 */
+    for (unsigned i = (unsigned)rec_i; i < (n - 1); ++i) {
+        or_square(c, limbs, nls, n, k);
+        if (c->last_max_diff > mx) mx = c->last_max_diff;
+        size_t max_sq = (2 * (size_t)n) / 64 + 4;
+        if (max_sq < nls) {
+            memset(limbs + max_sq, 0, (nls - max_sq) * sizeof(uint64_t));
+            for (size_t j = max_sq; j-- > (2 * (size_t)n) / 64 + 1;)
+                if (limbs[j] == 0xFFFFFFFFFFFFFFFFULL) limbs[j] = 0;
+        }
+        size_t cut = n >> 6;
+        size_t ss = last_hi * 2 + 5;
+        if (ss > nls) ss = nls;
+        size_t hi = highest_nonzero(limbs, ss);
+        last_hi = hi;
+        if (hi >= cut) or_mod_N(limbs, nls, n, k); /* stage1 + GMP fallback, by effect */
+        if (ckdir && i > 0 && i % 10000 == 0) or_save_checkpoint((uint64_t)i, nls, limbs, path);
+        ++iters;
+        if (stop_after && iters == stop_after) {
+            if (max_err_out) *max_err_out = mx;
+            if (iters_out) *iters_out = iters;
+            return -1;
+        }
+    }
+    int ok = or_is_nm1(limbs, nls, n, k);
+    if (ckdir) remove(path);
+    if (max_err_out) *max_err_out = mx;
+    if (iters_out) *iters_out = iters;
+    return ok;
+}

@@ -1,0 +1,107 @@
+"""Writes the golden fixtures under tests/golden/ with GMP (mpz_powm) as the
+arbiter.  Nothing here runs the reference; every value is a closed form of
+the Proth test:
+
+  residue after i squarings of the seed a^k  =  a^(k*2^i) mod N
+  Res64                                      =  that residue mod 2^64
+  verdict                                    =  a^((N-1)/2) == N-1 (mod N)
+
+with a chosen by the reference's base rule (P/proth_logic.cpp:423-435).
+
+  python tests/golden/make_golden.py small     # seconds
+  python tests/golden/make_golden.py kats      # ~5 min (n=120002 KATs)
+  python tests/golden/make_golden.py large     # ~10-20 min (n=2^21, 2^24)
+  python tests/golden/make_golden.py batch     # c4 subset, ~10 min
+"""
+from __future__ import annotations
+
+import json
+import random
+import sys
+import time
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent))
+sys.path.insert(0, str(HERE.parent.parent))
+
+import gmp_ctypes as G  # noqa: E402
+from prothlogic_amd import workloads as W  # noqa: E402
+
+M64 = (1 << 64) - 1
+
+
+def res_after(k, n, a, iters):
+    return G.residue_after(k, n, a, iters) & M64
+
+
+def case(k, n, iters_list, full=False):
+    a = G.reference_base(k, n)
+    out = {"k": k, "n": n, "a": a, "gated": W.gate_rejects(k, n), "res64_after": {}}
+    for i in iters_list:
+        t = time.time()
+        out["res64_after"][str(i)] = format(res_after(k, n, a, i), "016X")
+        print(f"  k={k} n={n} i={i} {time.time() - t:.1f}s", flush=True)
+    if full:
+        t = time.time()
+        prime, r64, _ = G.proth_verdict(k, n, a)
+        out["prime"] = bool(prime)
+        out["res64_final"] = format(r64, "016X")
+        print(f"  k={k} n={n} full {time.time() - t:.1f}s prime={prime}", flush=True)
+    return out
+
+
+def write(name, obj):
+    p = HERE / f"{name}.json"
+    p.write_text(json.dumps(obj, indent=1, sort_keys=True) + "\n")
+    print("wrote", p)
+
+
+def small():
+    rnd = random.Random(20261203)
+    cases = []
+    # Known small primes and composites, full verdicts.
+    for k, n in [(3, 20909), (3, 34350), (3, 34351), (5, 1947), (13, 3000), (1023, 32768)]:
+        cases.append(case(k, n, [1, 2, 10, 100], full=True))
+    # Random small candidates (ungated) across lengths the GPU supports.
+    for _ in range(24):
+        while True:
+            b = rnd.choice([3, 8, 12, 16, 20])
+            k = 2 * rnd.randrange(1, 1 << (b - 1)) + 1
+            n = rnd.choice([1000, 2048, 4097, 7001, 12345, 20000, 33333, 50000, 65536, 100003])
+            if not W.gate_rejects(k, n):
+                break
+        cases.append(case(k, n, [1, 3, 50, 500], full=n <= 20000))
+    write("small_cases", {"generator": "tests/golden/make_golden.py small", "cases": cases})
+
+
+def kats():
+    # The reference transcript's candidates (Run test:41,81,88) plus k=3 primes.
+    cases = []
+    for k, n in [(843, 120002), (853, 120002), (895, 120002)]:
+        cases.append(case(k, n, [1, 1000, 10000], full=True))
+    write("kats", {"generator": "tests/golden/make_golden.py kats", "cases": cases})
+
+
+def large():
+    cases = []
+    (k2, n2), = W.c2().candidates
+    cases.append(case(k2, n2, [1, 10, 100, 1000, 10000]))
+    (k3, n3), = W.c3().candidates
+    cases.append(case(k3, n3, [1, 10, 100, 1000]))
+    (k1, n1), = W.c1().candidates
+    cases.append(case(k1, n1, [1, 100, 10000], full=True))
+    write("large_cases", {"generator": "tests/golden/make_golden.py large", "cases": cases})
+
+
+def batch():
+    w = W.c4()
+    cases = [case(k, n, [200]) for (k, n) in w.candidates[:32]]
+    cases.append(case(3, 303093, [200]))
+    write("batch_cases", {"generator": "tests/golden/make_golden.py batch", "workload": w.name,
+                          "cases": cases})
+
+
+if __name__ == "__main__":
+    for what in sys.argv[1:] or ["small"]:
+        {"small": small, "kats": kats, "large": large, "batch": batch}[what]()
