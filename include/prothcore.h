@@ -1,0 +1,158 @@
+/*
+ * prothcore.h -- C ABI of the MI355X (gfx950) Proth squaring engine.
+ *
+ * Drop-in boundary for ProthLogic 4.2's squaring path.  Plain C: no C++
+ * types, no exceptions across the boundary, plain pointers and sizes.  Every
+ * function returns PL_OK (0) or a negative PL_ERR_* code; pl_last_error()
+ * gives a human-readable reason for the calling thread's last failure.
+ *
+ * The residue of a context stays on the device between calls; the host
+ * sees canonical residues only through pl_get_residue / pl_res64 /
+ * checkpoints.  A context owns one HIP stream (default priority) and is not
+ * safe to use from two threads at once; distinct contexts are independent.
+ *
+ * Each declaration names the reference interface it replaces
+ * (P/ = prothlogic4.2/ in the reference tree).
+ */
+#ifndef PROTHCORE_H
+#define PROTHCORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PL_OK 0
+#define PL_ERR_ARG (-1)      /* bad argument (sizes, k even, n out of range) */
+#define PL_ERR_HIP (-2)      /* HIP runtime failure or no usable GPU        */
+#define PL_ERR_ROUNDOFF (-3) /* FFT round-off above the guard: result void  */
+#define PL_ERR_STATE (-4)    /* device carry did not settle (internal bug)  */
+#define PL_ERR_IO (-5)       /* checkpoint file could not be read/written   */
+
+#define PL_ABI_VERSION 1
+
+typedef struct pl_ctx pl_ctx;
+
+typedef struct pl_info {
+    uint64_t k;
+    uint32_t n;
+    uint32_t fft_len;      /* complex points L (= digits of the residue)   */
+    uint32_t rows;         /* R: strided DFT length                         */
+    uint32_t cols;         /* C: contiguous DFT length                      */
+    double bits_per_digit; /* n / L                                         */
+    uint64_t device_bytes; /* HBM held by the context                       */
+} pl_info;
+
+/* Thread-local description of the last error (never NULL). */
+const char* pl_last_error(void);
+int pl_abi_version(void);
+/* Number of visible HIP devices (0 when none; not an error). */
+int pl_device_count(int* count);
+
+/* FFT-length rule (replaces auto_tune_fft_len, P/dcore.cpp:20-81): the
+ * smallest power-of-two L whose n/L keeps FP64 round-off within the guard
+ * and whose limb layout always holds the residue (no "no-fit" band). */
+int pl_plan_fft_len(uint32_t n, uint64_t k, uint32_t* fft_len);
+
+/* Context create/destroy (replaces fft_init / fft_destroy,
+ * P/fft_engine.h:10-11).  fft_len = 0 picks pl_plan_fft_len's length.
+ * The residue starts at 0. */
+int pl_ctx_create(uint64_t k, uint32_t n, uint32_t fft_len, int device, pl_ctx** out);
+void pl_ctx_destroy(pl_ctx* ctx);
+int pl_ctx_info(const pl_ctx* ctx, pl_info* out);
+
+/* Residue in/out as little-endian u64 limbs of the canonical value in
+ * [0, N) (the limb layout of P/proth_logic.cpp:442-446).  set reduces any
+ * value mod N; get zero-fills limbs above the value and fails with
+ * PL_ERR_ARG when n_limbs cannot hold it. */
+int pl_set_residue(pl_ctx* ctx, const uint64_t* limbs, size_t n_limbs);
+int pl_get_residue(pl_ctx* ctx, uint64_t* limbs, size_t n_limbs);
+
+/* x <- x^2 mod N, `iters` times, on the device (the loop body of
+ * P/proth_logic.cpp:485-549).  *max_err (nullable) gets the largest
+ * |s - rint(s)| over all iterations; err_trace (nullable, `iters` entries)
+ * gets each iteration's maximum.  Fails with PL_ERR_ROUNDOFF when an
+ * iteration exceeds `err_guard` (<= 0 selects the default 0.40); the
+ * residue is then unreliable. */
+int pl_square(pl_ctx* ctx, uint64_t iters, double err_guard, double* max_err, double* err_trace);
+
+/* x <- x*a mod N for a small a (< 2^32): the seed's powering step. */
+int pl_mul_small(pl_ctx* ctx, uint64_t a);
+
+/* Low 64 bits of the canonical residue (the build's Res64), and the
+ * verdict test x == N-1 (replaces is_nm1_limbs, P/proth_logic.cpp:336-355). */
+int pl_res64(pl_ctx* ctx, uint64_t* res64);
+int pl_is_n_minus_1(pl_ctx* ctx, int* is_nm1);
+
+/* Per-call compatibility shim with the exact semantics of
+ * fft_square_karatsuba(ctx, v, n_limbs, n_bits, k) (P/fft_engine.h:22,
+ * P/fft_engine.cpp:986-1073): v[0..n_limbs) <- v^2 mod (k*2^n_bits+1), and
+ * *last_max_diff (nullable) <- that squaring's max round-off
+ * (FFTContext::last_max_diff).  n_bits and k must match the context. */
+int pl_fft_square_karatsuba(pl_ctx* ctx, uint64_t* v, size_t n_limbs, unsigned n_bits, uint64_t k,
+                            double* last_max_diff);
+
+/* Diagnostics (no counterpart in the reference): average duration in ms of
+ * each kernel of one squaring, timed with HIP events on the context's own
+ * stream over `reps` launches: ms[0] forward column pass, ms[1] row pass
+ * (forward DFT, pointwise square, inverse DFT), ms[2] inverse column pass,
+ * ms[3] carry/fold, ms[4] block fix-up.  The residue is preserved.
+ * *traffic_bytes (nullable) gets the algorithmic HBM bytes one launch of
+ * each full-array kernel moves (read + write of the L complex points). */
+int pl_kernel_times(pl_ctx* ctx, int reps, double* ms, uint64_t* traffic_bytes);
+
+/* Small-prime gate (P/proth_logic.cpp:41-50, 405-408): 1 when some odd
+ * prime p <= 67 divides k*2^n+1. */
+int pl_gate_rejects(uint64_t k, uint32_t n);
+/* Base rule (P/proth_logic.cpp:423-435): first b of {2,3,5,...,37}, then
+ * 2..99, with Jacobi(b, N) = -1; 0 when none qualifies. */
+uint64_t pl_choose_base(uint64_t k, uint32_t n);
+
+typedef struct pl_test_args {
+    uint64_t k;
+    uint32_t n;
+    uint32_t fft_len;           /* 0: pl_plan_fft_len                           */
+    int device;
+    const char* checkpoint_path;/* NULL: no checkpoints; else this file         */
+    uint64_t checkpoint_every;  /* squarings between checkpoints (0: 10000)     */
+    uint64_t stop_after;        /* nonzero: stop after that many squarings      */
+    double err_guard;           /* <= 0: 0.40                                   */
+} pl_test_args;
+
+typedef struct pl_test_result {
+    int verdict;        /* 1 prime, 0 composite, -1 stopped early (stop_after) */
+    int gated;          /* 1 when the small-prime gate rejected the candidate  */
+    uint64_t base;      /* a                                                   */
+    uint64_t res64;     /* low 64 bits of a^(k*2^iters_done) mod N            */
+    uint64_t iters_done;/* squarings applied after the seed (n-1 when done)    */
+    uint64_t resumed_at;/* first squaring index run in this call               */
+    double max_err;     /* max round-off over the run                          */
+    uint32_t fft_len;
+} pl_test_result;
+
+/* Full Proth test a^((N-1)/2) mod N with the residue resident on the device
+ * (replaces execute_proth_test_fft, P/proth_logic.h:12 /
+ * P/proth_logic.cpp:384-580, and its checkpoint call sites :449-451,
+ * :536-538, :569, :576).  The checkpoint stores the next squaring index, so
+ * a resumed run applies exactly n-1 squarings in total. */
+int pl_proth_test(const pl_test_args* args, pl_test_result* out);
+
+/* Checkpoint files.  Format v2 (written): 64-byte header
+ *   {char magic[8]="PLCKPT02"; u64 k; u32 n; u32 base; u64 next_iter;
+ *    u64 n_limbs; u64 checksum(FNV-1a over limbs); u64 reserved}
+ * followed by n_limbs LE u64 limbs of the canonical residue.
+ * pl_load_checkpoint also reads the reference's format
+ * ({u64 i; size_t n_limbs} + limbs, P/checkpoint_manager.h:10-13) and maps
+ * its "last applied index" i to next_iter = i+1.  Replaces
+ * save_checkpoint / load_checkpoint (P/checkpoint_manager.h:15-16). */
+int pl_save_checkpoint(const char* path, uint64_t k, uint32_t n, uint64_t base, uint64_t next_iter,
+                       const uint64_t* limbs, size_t n_limbs);
+int pl_load_checkpoint(const char* path, uint64_t k, uint32_t n, uint64_t* base, uint64_t* next_iter,
+                       uint64_t* limbs, size_t n_limbs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PROTHCORE_H */

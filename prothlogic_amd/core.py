@@ -1,0 +1,227 @@
+"""ctypes binding of libprothcore.so (include/prothcore.h).
+
+Every call goes to the HIP engine; there is no CPU fallback.  Loading fails
+with an ImportError-like RuntimeError when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("PL_CORE_LIB", PKG / "libprothcore.so"))
+
+PL_OK = 0
+PL_ERR_ARG = -1
+PL_ERR_HIP = -2
+PL_ERR_ROUNDOFF = -3
+PL_ERR_STATE = -4
+PL_ERR_IO = -5
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class PlError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"prothcore error {code}: {msg}")
+        self.code = code
+
+
+class PlInfo(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint64), ("n", ctypes.c_uint32), ("fft_len", ctypes.c_uint32),
+                ("rows", ctypes.c_uint32), ("cols", ctypes.c_uint32), ("bits_per_digit", ctypes.c_double),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+class PlTestArgs(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint64), ("n", ctypes.c_uint32), ("fft_len", ctypes.c_uint32),
+                ("device", ctypes.c_int), ("checkpoint_path", ctypes.c_char_p),
+                ("checkpoint_every", ctypes.c_uint64), ("stop_after", ctypes.c_uint64),
+                ("err_guard", ctypes.c_double)]
+
+
+class PlTestResult(ctypes.Structure):
+    _fields_ = [("verdict", ctypes.c_int), ("gated", ctypes.c_int), ("base", ctypes.c_uint64),
+                ("res64", ctypes.c_uint64), ("iters_done", ctypes.c_uint64), ("resumed_at", ctypes.c_uint64),
+                ("max_err", ctypes.c_double), ("fft_len", ctypes.c_uint32)]
+
+
+_LIB = None
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m prothlogic_amd.build` (hipcc, gfx950)")
+    L = ctypes.CDLL(str(LIB_PATH))
+    vp, sz, u64, u32, i32, dbl = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_double
+    sig = {
+        "pl_last_error": ([], ctypes.c_char_p),
+        "pl_abi_version": ([], i32),
+        "pl_device_count": ([ctypes.POINTER(i32)], i32),
+        "pl_plan_fft_len": ([u32, u64, ctypes.POINTER(u32)], i32),
+        "pl_ctx_create": ([u64, u32, u32, i32, ctypes.POINTER(vp)], i32),
+        "pl_ctx_destroy": ([vp], None),
+        "pl_ctx_info": ([vp, ctypes.POINTER(PlInfo)], i32),
+        "pl_set_residue": ([vp, _u64p, sz], i32),
+        "pl_get_residue": ([vp, _u64p, sz], i32),
+        "pl_square": ([vp, u64, dbl, _dp, _dp], i32),
+        "pl_mul_small": ([vp, u64], i32),
+        "pl_res64": ([vp, _u64p], i32),
+        "pl_is_n_minus_1": ([vp, ctypes.POINTER(i32)], i32),
+        "pl_fft_square_karatsuba": ([vp, _u64p, sz, ctypes.c_uint, u64, _dp], i32),
+        "pl_kernel_times": ([vp, i32, _dp, _u64p], i32),
+        "pl_gate_rejects": ([u64, u32], i32),
+        "pl_choose_base": ([u64, u32], u64),
+        "pl_proth_test": ([ctypes.POINTER(PlTestArgs), ctypes.POINTER(PlTestResult)], i32),
+        "pl_save_checkpoint": ([ctypes.c_char_p, u64, u32, u64, u64, _u64p, sz], i32),
+        "pl_load_checkpoint": ([ctypes.c_char_p, u64, u32, _u64p, _u64p, _u64p, sz], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _LIB = L
+    return L
+
+
+def _check(rc: int):
+    if rc != PL_OK:
+        raise PlError(rc, lib().pl_last_error().decode(errors="replace"))
+
+
+def _limbs(x: int, nl: int):
+    arr = (ctypes.c_uint64 * nl)()
+    for i in range(nl):
+        arr[i] = (x >> (64 * i)) & 0xFFFFFFFFFFFFFFFF
+    return arr
+
+
+def _int(arr) -> int:
+    return int.from_bytes(bytes(arr), "little")
+
+
+def device_count() -> int:
+    c = ctypes.c_int(0)
+    _check(lib().pl_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def plan_fft_len(n: int, k: int) -> int:
+    L = ctypes.c_uint32(0)
+    _check(lib().pl_plan_fft_len(n, k, ctypes.byref(L)))
+    return L.value
+
+
+def gate_rejects(k: int, n: int) -> bool:
+    return bool(lib().pl_gate_rejects(k, n))
+
+
+def choose_base(k: int, n: int) -> int:
+    return int(lib().pl_choose_base(k, n))
+
+
+class Context:
+    """One candidate N = k*2^n+1 with its residue resident on the device."""
+
+    def __init__(self, k: int, n: int, fft_len: int = 0, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().pl_ctx_create(k, n, fft_len, device, ctypes.byref(h)))
+        self.h = h
+        self.k, self.n = k, n
+        self.N = k * (1 << n) + 1
+        self.nl = (n + 128) // 64 + 2
+        info = PlInfo()
+        _check(lib().pl_ctx_info(self.h, ctypes.byref(info)))
+        self.info = info
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pl_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def fft_len(self) -> int:
+        return self.info.fft_len
+
+    def set_residue(self, x: int):
+        nl = max(1, (x.bit_length() + 63) // 64)
+        _check(lib().pl_set_residue(self.h, _limbs(x, nl), nl))
+
+    def get_residue(self) -> int:
+        arr = (ctypes.c_uint64 * self.nl)()
+        _check(lib().pl_get_residue(self.h, arr, self.nl))
+        return _int(arr)
+
+    def square(self, iters: int, err_guard: float = 0.0, trace: bool = False):
+        e = ctypes.c_double(0.0)
+        tr = (ctypes.c_double * iters)() if trace and iters else None
+        _check(lib().pl_square(self.h, iters, err_guard, ctypes.byref(e), tr))
+        return (e.value, list(tr)) if trace else e.value
+
+    def mul_small(self, a: int):
+        _check(lib().pl_mul_small(self.h, a))
+
+    def res64(self) -> int:
+        r = ctypes.c_uint64(0)
+        _check(lib().pl_res64(self.h, ctypes.byref(r)))
+        return r.value
+
+    def is_n_minus_1(self) -> bool:
+        r = ctypes.c_int(0)
+        _check(lib().pl_is_n_minus_1(self.h, ctypes.byref(r)))
+        return bool(r.value)
+
+    KERNELS = ("fwd_col", "row_square", "inv_col", "carry", "fixup")
+
+    def kernel_times(self, reps: int = 20):
+        """{kernel: ms per launch} timed with HIP events on the context's stream,
+        plus the algorithmic bytes of one full-array pass."""
+        ms = (ctypes.c_double * 5)()
+        tb = ctypes.c_uint64(0)
+        _check(lib().pl_kernel_times(self.h, reps, ms, ctypes.byref(tb)))
+        return dict(zip(self.KERNELS, list(ms))), tb.value
+
+    def fft_square_karatsuba(self, limbs, n_limbs: int):
+        arr = (ctypes.c_uint64 * n_limbs)(*limbs)
+        e = ctypes.c_double(0.0)
+        _check(lib().pl_fft_square_karatsuba(self.h, arr, n_limbs, self.n, self.k, ctypes.byref(e)))
+        return list(arr), e.value
+
+
+def proth_test(k: int, n: int, fft_len: int = 0, device: int = 0, checkpoint_path: str | None = None,
+               checkpoint_every: int = 0, stop_after: int = 0, err_guard: float = 0.0) -> dict:
+    a = PlTestArgs(k, n, fft_len, device, checkpoint_path.encode() if checkpoint_path else None,
+                   checkpoint_every, stop_after, err_guard)
+    r = PlTestResult()
+    _check(lib().pl_proth_test(ctypes.byref(a), ctypes.byref(r)))
+    return dict(verdict=r.verdict, gated=bool(r.gated), base=r.base, res64=r.res64, iters_done=r.iters_done,
+                resumed_at=r.resumed_at, max_err=r.max_err, fft_len=r.fft_len)
+
+
+def save_checkpoint(path: str, k: int, n: int, base: int, next_iter: int, x: int, nl: int):
+    _check(lib().pl_save_checkpoint(path.encode(), k, n, base, next_iter, _limbs(x, nl), nl))
+
+
+def load_checkpoint(path: str, k: int, n: int, nl: int):
+    b = ctypes.c_uint64(0)
+    it = ctypes.c_uint64(0)
+    arr = (ctypes.c_uint64 * nl)()
+    _check(lib().pl_load_checkpoint(path.encode(), k, n, ctypes.byref(b), ctypes.byref(it), arr, nl))
+    return b.value, it.value, _int(arr)

@@ -1,0 +1,485 @@
+// capi.cpp -- the C ABI of include/prothcore.h over pleng::Engine and the
+// host bignum helpers.  No exception crosses this boundary.
+#include "prothcore.h"
+
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bignum.h"
+#include "engine.h"
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const std::invalid_argument& e) {
+        return fail(PL_ERR_ARG, e.what());
+    } catch (const std::exception& e) {
+        return fail(PL_ERR_HIP, e.what());
+    }
+}
+
+using u128 = unsigned __int128;
+
+uint64_t mulmod(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)(((u128)a * b) % m); }
+
+uint64_t powmod(uint64_t a, uint64_t e, uint64_t m)
+{
+    uint64_t r = 1 % m;
+    a %= m;
+    while (e) {
+        if (e & 1) r = mulmod(r, a, m);
+        a = mulmod(a, a, m);
+        e >>= 1;
+    }
+    return r;
+}
+
+int jacobi_u64(uint64_t a, uint64_t m)  // m odd
+{
+    int t = 1;
+    a %= m;
+    while (a) {
+        while ((a & 1) == 0) {
+            a >>= 1;
+            uint64_t r = m & 7;
+            if (r == 3 || r == 5) t = -t;
+        }
+        std::swap(a, m);
+        if ((a & 3) == 3 && (m & 3) == 3) t = -t;
+        a %= m;
+    }
+    return m == 1 ? t : 0;
+}
+
+// Jacobi(b, k*2^n+1) for small b by reciprocity (N mod 8 for the factors of
+// two, N mod b' for the odd part; (N-1)/2 = k*2^(n-1) is even for n >= 2).
+int jacobi_small(uint64_t b, uint64_t k, uint32_t n)
+{
+    const uint64_t n8 = ((k % 8) * powmod(2, n, 8) + 1) % 8;
+    int t = 1;
+    while (b && (b & 1) == 0) {
+        b >>= 1;
+        if (n8 == 3 || n8 == 5) t = -t;
+    }
+    if (b == 0) return 0;
+    if (b == 1) return t;
+    const uint64_t nmod = ((k % b) * powmod(2, n, b) + 1) % b;
+    if (n == 1 && (k & 1) && ((b - 1) / 2) % 2 == 1) t = -t;
+    if (nmod == 0) return 0;
+    return t * jacobi_u64(nmod, b);
+}
+
+
+}  // namespace
+
+struct pl_ctx {
+    pleng::Plan plan;
+    std::unique_ptr<pleng::Engine> eng;
+    std::unique_ptr<plbn::Modulus> M;
+    int device = 0;
+
+    plbn::Limbs mont_value()  // canonical Montgomery form Y of the device residue
+    {
+        std::vector<int32_t> d;
+        int64_t xhi = 0;
+        eng->download(d, xhi);
+        return plbn::from_digits(*M, plan.L, d, xhi);
+    }
+    void set_mont_value(const plbn::Limbs& y)
+    {
+        std::vector<int32_t> d;
+        int64_t xhi = 0;
+        plbn::to_digits(*M, plan.L, y, d, xhi);
+        eng->upload(d, xhi);
+    }
+    plbn::Limbs value() { return plbn::from_mont(*M, mont_value()); }
+    void set_value(const plbn::Limbs& x) { set_mont_value(plbn::to_mont(*M, x)); }
+};
+
+extern "C" {
+
+const char* pl_last_error(void) { return g_err.c_str(); }
+
+int pl_abi_version(void) { return PL_ABI_VERSION; }
+
+int pl_device_count(int* count)
+{
+    if (!count) return fail(PL_ERR_ARG, "count is NULL");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return PL_OK;
+}
+
+int pl_plan_fft_len(uint32_t n, uint64_t k, uint32_t* fft_len)
+{
+    if (!fft_len) return fail(PL_ERR_ARG, "fft_len is NULL");
+    pleng::Plan p;
+    std::string why;
+    if (!pleng::choose_plan(n, k, 0, p, why)) return fail(PL_ERR_ARG, why);
+    *fft_len = p.L;
+    return PL_OK;
+}
+
+int pl_ctx_create(uint64_t k, uint32_t n, uint32_t fft_len, int device, pl_ctx** out)
+{
+    if (!out) return fail(PL_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    return guarded([&] {
+        auto c = std::make_unique<pl_ctx>();
+        std::string why;
+        if (!pleng::choose_plan(n, k, fft_len, c->plan, why)) return fail(PL_ERR_ARG, why);
+        int nd = 0;
+        if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return fail(PL_ERR_HIP, "no HIP device visible");
+        if (device < 0 || device >= nd) return fail(PL_ERR_ARG, "device index out of range");
+        c->device = device;
+        c->M = std::make_unique<plbn::Modulus>(n, k);
+        c->eng = std::make_unique<pleng::Engine>(c->plan, device);
+        *out = c.release();
+        return PL_OK;
+    });
+}
+
+void pl_ctx_destroy(pl_ctx* ctx) { delete ctx; }
+
+int pl_ctx_info(const pl_ctx* ctx, pl_info* out)
+{
+    if (!ctx || !out) return fail(PL_ERR_ARG, "NULL argument");
+    out->k = ctx->plan.k;
+    out->n = ctx->plan.n;
+    out->fft_len = ctx->plan.L;
+    out->rows = ctx->plan.R;
+    out->cols = ctx->plan.C;
+    out->bits_per_digit = (double)ctx->plan.n / (double)ctx->plan.L;
+    out->device_bytes = ctx->eng->device_bytes();
+    return PL_OK;
+}
+
+int pl_set_residue(pl_ctx* ctx, const uint64_t* limbs, size_t n_limbs)
+{
+    if (!ctx || (!limbs && n_limbs)) return fail(PL_ERR_ARG, "NULL argument");
+    return guarded([&] {
+        plbn::Limbs x(limbs, limbs + n_limbs);
+        if (x.empty()) x.push_back(0);
+        ctx->set_value(plbn::mod_N(*ctx->M, x));
+        return PL_OK;
+    });
+}
+
+int pl_get_residue(pl_ctx* ctx, uint64_t* limbs, size_t n_limbs)
+{
+    if (!ctx || !limbs) return fail(PL_ERR_ARG, "NULL argument");
+    return guarded([&] {
+        plbn::Limbs x = ctx->value();
+        size_t used = x.size();
+        while (used > 0 && x[used - 1] == 0) --used;
+        if (used > n_limbs) return fail(PL_ERR_ARG, "n_limbs too small for the residue");
+        for (size_t i = 0; i < n_limbs; ++i) limbs[i] = i < x.size() ? x[i] : 0;
+        return PL_OK;
+    });
+}
+
+int pl_square(pl_ctx* ctx, uint64_t iters, double err_guard, double* max_err, double* err_trace)
+{
+    if (!ctx) return fail(PL_ERR_ARG, "NULL context");
+    if (err_guard <= 0) err_guard = 0.40;
+    return guarded([&] {
+        double e = ctx->eng->square(iters, err_trace);
+        if (max_err) *max_err = e;
+        int st = ctx->eng->status();
+        if (st) return fail(PL_ERR_STATE, "device carry did not settle (status " + std::to_string(st) + ")");
+        if (e > err_guard)
+            return fail(PL_ERR_ROUNDOFF, "round-off " + std::to_string(e) + " above guard " + std::to_string(err_guard));
+        return PL_OK;
+    });
+}
+
+int pl_mul_small(pl_ctx* ctx, uint64_t a)
+{
+    if (!ctx) return fail(PL_ERR_ARG, "NULL context");
+    return guarded([&] {
+        ctx->set_mont_value(plbn::mul_small_mod(*ctx->M, ctx->mont_value(), a));
+        return PL_OK;
+    });
+}
+
+int pl_res64(pl_ctx* ctx, uint64_t* res64)
+{
+    if (!ctx || !res64) return fail(PL_ERR_ARG, "NULL argument");
+    return guarded([&] {
+        *res64 = plbn::low64(ctx->value());
+        return PL_OK;
+    });
+}
+
+int pl_is_n_minus_1(pl_ctx* ctx, int* is_nm1)
+{
+    if (!ctx || !is_nm1) return fail(PL_ERR_ARG, "NULL argument");
+    return guarded([&] {
+        *is_nm1 = plbn::is_n_minus_1(*ctx->M, ctx->value()) ? 1 : 0;
+        return PL_OK;
+    });
+}
+
+int pl_fft_square_karatsuba(pl_ctx* ctx, uint64_t* v, size_t n_limbs, unsigned n_bits, uint64_t k,
+                            double* last_max_diff)
+{
+    if (!ctx || !v) return fail(PL_ERR_ARG, "NULL argument");
+    if (n_bits != ctx->plan.n || k != ctx->plan.k) return fail(PL_ERR_ARG, "n_bits/k differ from the context");
+    int rc = pl_set_residue(ctx, v, n_limbs);
+    if (rc) return rc;
+    double e = 0;
+    rc = pl_square(ctx, 1, 1.0, &e, nullptr);
+    if (rc) return rc;
+    if (last_max_diff) *last_max_diff = e;
+    return pl_get_residue(ctx, v, n_limbs);
+}
+
+int pl_kernel_times(pl_ctx* ctx, int reps, double* ms, uint64_t* traffic_bytes)
+{
+    if (!ctx || !ms || reps <= 0) return fail(PL_ERR_ARG, "bad argument");
+    return guarded([&] {
+        ctx->eng->kernel_times(reps, ms);
+        if (traffic_bytes) *traffic_bytes = 32ull * ctx->plan.L;
+        return PL_OK;
+    });
+}
+
+int pl_gate_rejects(uint64_t k, uint32_t n)
+{
+    static const uint64_t ps[] = { 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67 };
+    for (uint64_t p : ps) {
+        uint64_t km = k % p;
+        if (km == 0) continue;
+        if ((km * powmod(2, n, p)) % p == p - 1) return 1;
+    }
+    return 0;
+}
+
+uint64_t pl_choose_base(uint64_t k, uint32_t n)
+{
+    static const uint64_t pref[] = { 2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37 };
+    for (uint64_t b : pref)
+        if (jacobi_small(b, k, n) == -1) return b;
+    for (uint64_t b = 2; b < 100; ++b)
+        if (jacobi_small(b, k, n) == -1) return b;
+    return 0;
+}
+
+// ------------------------------------------------------------ checkpoints
+
+static uint64_t fnv1a(const uint64_t* p, size_t n)
+{
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n * 8; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+struct CkptV2 {
+    char magic[8];
+    uint64_t k;
+    uint32_t n;
+    uint32_t base;
+    uint64_t next_iter;
+    uint64_t n_limbs;
+    uint64_t checksum;
+    uint64_t reserved;
+};
+static_assert(sizeof(CkptV2) == 56 || sizeof(CkptV2) == 64, "header layout");
+
+int pl_save_checkpoint(const char* path, uint64_t k, uint32_t n, uint64_t base, uint64_t next_iter,
+                       const uint64_t* limbs, size_t n_limbs)
+{
+    if (!path || !limbs) return fail(PL_ERR_ARG, "NULL argument");
+    unsigned char hdr[64] = { 0 };
+    CkptV2 h;
+    std::memcpy(h.magic, "PLCKPT02", 8);
+    h.k = k;
+    h.n = n;
+    h.base = (uint32_t)base;
+    h.next_iter = next_iter;
+    h.n_limbs = n_limbs;
+    h.checksum = fnv1a(limbs, n_limbs);
+    h.reserved = 0;
+    std::memcpy(hdr, &h, sizeof h);
+    std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return fail(PL_ERR_IO, "cannot open " + tmp);
+    bool ok = std::fwrite(hdr, 1, 64, f) == 64 && std::fwrite(limbs, 8, n_limbs, f) == n_limbs;
+    ok = ok && std::fflush(f) == 0;
+    std::fclose(f);
+    if (!ok) return fail(PL_ERR_IO, "short write to " + tmp);
+    if (std::rename(tmp.c_str(), path) != 0) return fail(PL_ERR_IO, "rename failed for " + tmp);
+    return PL_OK;
+}
+
+int pl_load_checkpoint(const char* path, uint64_t k, uint32_t n, uint64_t* base, uint64_t* next_iter,
+                       uint64_t* limbs, size_t n_limbs)
+{
+    if (!path || !limbs || !next_iter) return fail(PL_ERR_ARG, "NULL argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(PL_ERR_IO, std::string("cannot open ") + path);
+    std::fseek(f, 0, SEEK_END);
+    long size = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    unsigned char hdr[64] = { 0 };
+    size_t got = std::fread(hdr, 1, 64, f);
+    int rc = PL_OK;
+    if (got >= 16 && std::memcmp(hdr, "PLCKPT02", 8) == 0 && got == 64) {
+        CkptV2 h;
+        std::memcpy(&h, hdr, sizeof h);
+        if (h.k != k || h.n != n) rc = fail(PL_ERR_IO, "checkpoint is for another candidate");
+        else if ((long)(64 + 8 * h.n_limbs) != size) rc = fail(PL_ERR_IO, "checkpoint truncated");
+        else {
+            std::vector<uint64_t> buf(h.n_limbs);
+            std::fseek(f, 64, SEEK_SET);
+            if (std::fread(buf.data(), 8, h.n_limbs, f) != h.n_limbs) rc = fail(PL_ERR_IO, "short read");
+            else if (fnv1a(buf.data(), buf.size()) != h.checksum) rc = fail(PL_ERR_IO, "checkpoint checksum mismatch");
+            else {
+                for (size_t i = 0; i < n_limbs; ++i) limbs[i] = i < buf.size() ? buf[i] : 0;
+                for (size_t i = n_limbs; i < buf.size(); ++i)
+                    if (buf[i]) rc = fail(PL_ERR_ARG, "n_limbs too small for the checkpoint");
+                if (rc == PL_OK) {
+                    *next_iter = h.next_iter;
+                    if (base) *base = h.base;
+                }
+            }
+        }
+    } else if (got >= 16) {
+        // Reference layout {u64 current_i; size_t n_limbs} + limbs; current_i
+        // is the index of the last squaring applied (P/proth_logic.cpp:536-537).
+        uint64_t cur_i, nl;
+        std::memcpy(&cur_i, hdr, 8);
+        std::memcpy(&nl, hdr + 8, 8);
+        if ((long)(16 + 8 * nl) != size) rc = fail(PL_ERR_IO, "reference checkpoint truncated");
+        else {
+            std::vector<uint64_t> buf(nl);
+            std::fseek(f, 16, SEEK_SET);
+            if (std::fread(buf.data(), 8, nl, f) != nl) rc = fail(PL_ERR_IO, "short read");
+            else {
+                for (size_t i = 0; i < n_limbs; ++i) limbs[i] = i < buf.size() ? buf[i] : 0;
+                for (size_t i = n_limbs; i < buf.size(); ++i)
+                    if (buf[i]) rc = fail(PL_ERR_ARG, "n_limbs too small for the checkpoint");
+                if (rc == PL_OK) {
+                    *next_iter = cur_i + 1;
+                    if (base) *base = 0;
+                }
+            }
+        }
+    } else {
+        rc = fail(PL_ERR_IO, "checkpoint header truncated");
+    }
+    std::fclose(f);
+    return rc;
+}
+
+// ------------------------------------------------------------ full test
+
+int pl_proth_test(const pl_test_args* args, pl_test_result* out)
+{
+    if (!args || !out) return fail(PL_ERR_ARG, "NULL argument");
+    std::memset(out, 0, sizeof *out);
+    const uint64_t k = args->k;
+    const uint32_t n = args->n;
+    if (pl_gate_rejects(k, n)) {
+        out->gated = 1;
+        out->verdict = 0;
+        return PL_OK;
+    }
+    const uint64_t a = pl_choose_base(k, n);
+    out->base = a;
+    pl_ctx* raw = nullptr;
+    int rc = pl_ctx_create(k, n, args->fft_len, args->device, &raw);
+    if (rc) return rc;
+    std::unique_ptr<pl_ctx> ctx(raw);
+    out->fft_len = ctx->plan.L;
+    const double guard = args->err_guard > 0 ? args->err_guard : 0.40;
+    const uint64_t every = args->checkpoint_every ? args->checkpoint_every : 10000;
+    const uint64_t total = n - 1;
+    const size_t nl = ctx->M->nl;
+    return guarded([&] {
+        uint64_t next = 0;
+        bool resumed = false;
+        if (args->checkpoint_path) {
+            std::vector<uint64_t> buf(nl, 0);
+            uint64_t b = 0, nx = 0;
+            FILE* probe = std::fopen(args->checkpoint_path, "rb");
+            if (probe) {
+                std::fclose(probe);
+                int lr = pl_load_checkpoint(args->checkpoint_path, k, n, &b, &nx, buf.data(), nl);
+                if (lr != PL_OK) return lr;  // a corrupt or foreign file is an error, not a silent restart
+                if (nx > total) return fail(PL_ERR_IO, "checkpoint iteration beyond n-1");
+                ctx->set_value(plbn::Limbs(buf.begin(), buf.end()));
+                next = nx;
+                resumed = true;
+            }
+        }
+        if (!resumed) {
+            // seed a^k mod N, left to right, Montgomery form on the device
+            if (a == 0) {
+                ctx->set_value(plbn::Limbs(1, 0));
+            } else {
+                ctx->set_value(plbn::Limbs(1, a));
+                int top = 63;
+                while (top > 0 && !((k >> top) & 1)) --top;
+                for (int bit = top - 1; bit >= 0; --bit) {
+                    double e = ctx->eng->square(1, nullptr);
+                    if (e > out->max_err) out->max_err = e;
+                    if ((k >> bit) & 1) ctx->set_mont_value(plbn::mul_small_mod(*ctx->M, ctx->mont_value(), a));
+                }
+            }
+        }
+        out->resumed_at = next;
+        while (next < total) {
+            uint64_t chunk = std::min<uint64_t>(every - (next % every), total - next);
+            if (args->stop_after && out->iters_done + chunk > args->stop_after)
+                chunk = args->stop_after - out->iters_done;
+            if (chunk == 0) break;
+            double e = ctx->eng->square(chunk, nullptr);
+            if (e > out->max_err) out->max_err = e;
+            int st = ctx->eng->status();
+            if (st) return fail(PL_ERR_STATE, "device carry did not settle");
+            if (e > guard) return fail(PL_ERR_ROUNDOFF, "round-off " + std::to_string(e) + " above guard");
+            next += chunk;
+            out->iters_done += chunk;
+            if (args->checkpoint_path && next < total && (next % every == 0 || args->stop_after)) {
+                plbn::Limbs x = ctx->value();
+                int s = pl_save_checkpoint(args->checkpoint_path, k, n, a, next, x.data(), x.size());
+                if (s) return s;
+            }
+            if (args->stop_after && out->iters_done >= args->stop_after) break;
+        }
+        plbn::Limbs x = ctx->value();
+        out->res64 = plbn::low64(x);
+        if (next < total) {
+            out->verdict = -1;
+            return PL_OK;
+        }
+        out->verdict = plbn::is_n_minus_1(*ctx->M, x) ? 1 : 0;
+        if (args->checkpoint_path) std::remove(args->checkpoint_path);
+        return PL_OK;
+    });
+}
+
+}  // extern "C"
