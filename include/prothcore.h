@@ -43,6 +43,8 @@ typedef struct pl_info {
     uint32_t cols;         /* C: contiguous DFT length                      */
     double bits_per_digit; /* n / L                                         */
     uint64_t device_bytes; /* HBM held by the context                       */
+    uint32_t passes;       /* full-array kernel passes per steady-state squaring:
+                              2 with the fused outer pass, 4 without        */
 } pl_info;
 
 /* Thread-local description of the last error (never NULL). */
@@ -58,7 +60,8 @@ int pl_plan_fft_len(uint32_t n, uint64_t k, uint32_t* fft_len);
 
 /* Context create/destroy (replaces fft_init / fft_destroy,
  * P/fft_engine.h:10-11).  fft_len = 0 picks pl_plan_fft_len's length.
- * The residue starts at 0. */
+ * The residue starts at 0.  The environment variable PL_FUSED=0 disables
+ * the fused outer pass (4 kernel passes per squaring instead of 2). */
 int pl_ctx_create(uint64_t k, uint32_t n, uint32_t fft_len, int device, pl_ctx** out);
 void pl_ctx_destroy(pl_ctx* ctx);
 int pl_ctx_info(const pl_ctx* ctx, pl_info* out);
@@ -96,9 +99,11 @@ int pl_fft_square_karatsuba(pl_ctx* ctx, uint64_t* v, size_t n_limbs, unsigned n
 
 /* Diagnostics (no counterpart in the reference): average duration in ms of
  * each kernel of one squaring, timed with HIP events on the context's own
- * stream over `reps` launches: ms[0] forward column pass, ms[1] row pass
- * (forward DFT, pointwise square, inverse DFT), ms[2] inverse column pass,
- * ms[3] carry/fold, ms[4] block fix-up.  The residue is preserved.
+ * stream over `reps` launches, into ms[0..5]: ms[0] forward column pass,
+ * ms[1] row pass (forward DFT, pointwise square, inverse DFT), ms[2] inverse
+ * column pass, ms[3] flat carry/fold, ms[4] block fix-up, ms[5] fused outer
+ * pass (inverse column DFT + carry/fold + forward column DFT; 0 when the
+ * context does not use it).  The residue is preserved.
  * *traffic_bytes (nullable) gets the algorithmic HBM bytes one launch of
  * each full-array kernel moves (read + write of the L complex points). */
 int pl_kernel_times(pl_ctx* ctx, int reps, double* ms, uint64_t* traffic_bytes);

@@ -32,7 +32,7 @@ class PlError(RuntimeError):
 class PlInfo(ctypes.Structure):
     _fields_ = [("k", ctypes.c_uint64), ("n", ctypes.c_uint32), ("fft_len", ctypes.c_uint32),
                 ("rows", ctypes.c_uint32), ("cols", ctypes.c_uint32), ("bits_per_digit", ctypes.c_double),
-                ("device_bytes", ctypes.c_uint64)]
+                ("device_bytes", ctypes.c_uint64), ("passes", ctypes.c_uint32)]
 
 
 class PlTestArgs(ctypes.Structure):
@@ -188,12 +188,12 @@ class Context:
         _check(lib().pl_is_n_minus_1(self.h, ctypes.byref(r)))
         return bool(r.value)
 
-    KERNELS = ("fwd_col", "row_square", "inv_col", "carry", "fixup")
+    KERNELS = ("fwd_col", "row_square", "inv_col", "carry", "fixup", "outer")
 
     def kernel_times(self, reps: int = 20):
         """{kernel: ms per launch} timed with HIP events on the context's stream,
         plus the algorithmic bytes of one full-array pass."""
-        ms = (ctypes.c_double * 5)()
+        ms = (ctypes.c_double * 6)()
         tb = ctypes.c_uint64(0)
         _check(lib().pl_kernel_times(self.h, reps, ms, ctypes.byref(tb)))
         return dict(zip(self.KERNELS, list(ms))), tb.value

@@ -3,6 +3,7 @@
 #include "prothcore.h"
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -151,7 +152,10 @@ int pl_ctx_create(uint64_t k, uint32_t n, uint32_t fft_len, int device, pl_ctx**
         if (device < 0 || device >= nd) return fail(PL_ERR_ARG, "device index out of range");
         c->device = device;
         c->M = std::make_unique<plbn::Modulus>(n, k);
-        c->eng = std::make_unique<pleng::Engine>(c->plan, device);
+        const char* fz = std::getenv("PL_FUSED");
+        const bool allow_fused = !(fz && fz[0] == '0');
+        c->eng = std::make_unique<pleng::Engine>(c->plan, device, allow_fused);
+        c->plan = c->eng->plan();
         *out = c.release();
         return PL_OK;
     });
@@ -169,6 +173,7 @@ int pl_ctx_info(const pl_ctx* ctx, pl_info* out)
     out->cols = ctx->plan.C;
     out->bits_per_digit = (double)ctx->plan.n / (double)ctx->plan.L;
     out->device_bytes = ctx->eng->device_bytes();
+    out->passes = ctx->plan.fused ? 2 : 4;
     return PL_OK;
 }
 

@@ -16,7 +16,8 @@ struct Plan {
     uint32_t n = 0;
     uint64_t k = 0;
     uint32_t L = 0, logL = 0, R = 0, C = 0;
-    uint32_t s = 0;  // log2 of the low half of the two-level tables
+    uint32_t s = 0;      // log2 of the low half of the two-level tables
+    bool fused = false;  // steady state uses the fused outer pass (2 kernels per squaring)
 };
 
 // Rules for L: smallest power of two in [2^10, 2^22] whose bits per digit
@@ -26,10 +27,11 @@ bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::strin
 double max_bits_per_digit(uint32_t logL);
 
 struct DeviceTables;
+struct Params;
 
 class Engine {
   public:
-    Engine(const Plan& p, int device);
+    Engine(const Plan& p, int device, bool allow_fused = true);
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
@@ -49,18 +51,19 @@ class Engine {
     // device status word (non-zero: a carry did not settle within a run)
     int status();
 
-    // Average duration (ms) of each kernel of one squaring, timed with HIP
-    // events on this engine's stream over `reps` launches each:
-    // [0] forward column pass, [1] row pass (DFT, square, inverse DFT),
-    // [2] inverse column pass, [3] carry, [4] block fix-up.  The residue is
-    // saved before and restored after.
-    void kernel_times(int reps, double* ms5);
-    static constexpr int kNumKernels = 5;
+    // Average duration (ms) of each kernel, timed with HIP events on this
+    // engine's stream over `reps` launches each: [0] forward column pass,
+    // [1] row pass (DFT, square, inverse DFT), [2] inverse column pass,
+    // [3] flat carry, [4] block fix-up, [5] fused outer pass (0 when the
+    // plan does not use it).  The residue is saved before and restored after.
+    void kernel_times(int reps, double* ms);
+    static constexpr int kNumKernels = 6;
 
     static constexpr int kGraphChunk = 64;
 
   private:
-    void enqueue_one(uint64_t slot);
+    Params params(uint64_t slot) const;
+    void enqueue_chunk(int g);
     void build_graph(int iters);
 
     Plan plan_;
@@ -70,8 +73,11 @@ class Engine {
     void* mem_ = nullptr;
     double2* z_ = nullptr;
     long long* carries_ = nullptr;
+    long long* runc_ = nullptr;
+    unsigned* flags_ = nullptr;
+    unsigned* ticket_ = nullptr;
     long long* xhi_ = nullptr;
-    unsigned long long* err_ = nullptr;   // kGraphChunk slots (bit patterns of non-negative doubles)
+    unsigned long long* err_ = nullptr;  // kGraphChunk slots (bit patterns of non-negative doubles)
     int* status_ = nullptr;
     int32_t* digits_dev_ = nullptr;
     DeviceTables* tabs_ = nullptr;

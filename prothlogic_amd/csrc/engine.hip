@@ -1,4 +1,4 @@
-// engine.hip -- CDNA4 (gfx950) kernels for one modular squaring of a Proth
+// engine.hip -- CDNA4 (gfx950) kernels for the modular squaring of a Proth
 // residue, N = k*2^n + 1, kept on the device for the whole test.
 //
 // Representation.  The residue is x = X_lo + X_hi*2^n where X_lo has L
@@ -21,18 +21,24 @@
 // squaring keep |x| below ~k*2^(n-1); the residue is therefore held in the
 // Montgomery form y = x*2^(2n) mod N (conversions happen on the host, at the
 // start and end of a test only).  The three carry passes (T_lo, the first fold
-// S, the second fold R) run per run of RUN consecutive digits in int64; each
-// run publishes its own three carry-outs, computed from its own digits only,
-// and the next run folds them into its first digits (one hop, no chain).
+// S, the second fold R) run per run of consecutive digits in int64; each run
+// publishes its own three carry-outs, computed from its own digits only, and
+// the next run folds them into its first digits (one hop, no chain).
 //
-// FFT.  L = R*C, four-step: R-point DFTs down the columns (stride C) with the
-// W_L^(c*rho) twiddle, then C-point DFTs along rows; the frequency data stays
-// in that transposed order, the pointwise square runs in the row kernel
-// between the forward and inverse row DFTs, and the inverse column kernel
-// undoes the first step.  Each DFT is a Stockham radix-8/4/2 autosort staged
-// in LDS.
+// FFT.  L = R*C, four-step on the R x C row-major array (digit m at r*C + c):
+// R-point DFTs down the columns with the W_L^(c*rho) twiddle, then C-point
+// DFTs along rows; the spectrum stays in that transposed order.  One squaring
+// is two kernels in steady state:
+//   k_outer : inverse column DFT -> unweight/round/carry/fold -> weight ->
+//             forward column DFT (block of NT columns, runs of NT digits)
+//   k_row   : forward row DFT -> Z*(Z + 2i*X_hi) -> inverse row DFT
+// k_fwd_col / k_inv_col / k_carry / k_fixup are the unfused pieces used to
+// enter and leave the steady state (and for lengths the fused path does not
+// cover).
 #include "engine.h"
+#include "fft_lds.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -56,7 +62,10 @@ struct Params {
     const double* ipw_hi;   // 2^(-j*2^s/L) / L
     const double2* om_lo;   // exp(i*pi*j/(2L))
     const double2* om_hi;   // exp(i*pi*j*2^s/(2L))
-    long long* carries;     // 3 per carry block: own carry-outs of its last run
+    long long* carries;     // flat carry kernel: 3 per block (own carry-outs of its last run)
+    long long* runc;        // outer kernel: 3 per run (block-major, then row)
+    unsigned* flags;        // outer kernel: per-block publish counters
+    unsigned* ticket;       // outer kernel: logical block ticket counter
     long long* xhi;
     unsigned long long* err;
     int* status;
@@ -72,13 +81,6 @@ struct DeviceTables {
 };
 
 // ------------------------------------------------------------- helpers
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
-{
-    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
 __device__ __forceinline__ double2 twL(const Params& p, unsigned j)  // W_L^j, j < L
 {
@@ -123,212 +125,43 @@ __device__ __forceinline__ long long bal(long long v, int b, long long& carry)
     return v - q * (1ll << b);
 }
 
-// ------------------------------------------------------------- butterflies
-
-template <bool INV>
-__device__ __forceinline__ double2 mul_mi(double2 a)  // forward: -i*a, inverse: +i*a
+// One digit through the three carry passes: T_lo, S = T_hi - k*T_lo,
+// R = S_hi - k*S_lo.  Returns the digit of R; advances the carries.
+__device__ __forceinline__ long long carry3(double tr, double hr, long long k, int b, long long& q1, long long& q2,
+                                           long long& q3)
 {
-    return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+    const long long d1 = bal((long long)tr + q1, b, q1);
+    const long long d2 = bal((long long)hr - k * d1 + q2, b, q2);
+    return bal(-k * d2 + q3, b, q3);
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft2(double2* v)
-{
-    double2 a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-}
-
-template <bool INV>
-__device__ __forceinline__ void dft4(double2* v)
-{
-    double2 s0 = cadd(v[0], v[2]), s1 = csub(v[0], v[2]);
-    double2 s2 = cadd(v[1], v[3]), s3 = csub(v[1], v[3]);
-    double2 d = mul_mi<INV>(s3);
-    v[0] = cadd(s0, s2);
-    v[1] = cadd(s1, d);
-    v[2] = csub(s0, s2);
-    v[3] = csub(s1, d);
-}
-
-template <bool INV>
-__device__ __forceinline__ void dft8(double2* v)
-{
-    constexpr double h = 0.70710678118654752440;
-    constexpr double sg = INV ? -1.0 : 1.0;
-    double2 e[4] = { v[0], v[2], v[4], v[6] };
-    double2 o[4] = { v[1], v[3], v[5], v[7] };
-    dft4<INV>(e);
-    dft4<INV>(o);
-    double2 t1 = make_double2(h * (o[1].x + sg * o[1].y), h * (o[1].y - sg * o[1].x));
-    double2 t2 = mul_mi<INV>(o[2]);
-    double2 t3 = make_double2(h * (-o[3].x + sg * o[3].y), h * (-o[3].y - sg * o[3].x));
-    v[0] = cadd(e[0], o[0]);
-    v[4] = csub(e[0], o[0]);
-    v[1] = cadd(e[1], t1);
-    v[5] = csub(e[1], t1);
-    v[2] = cadd(e[2], t2);
-    v[6] = csub(e[2], t2);
-    v[3] = cadd(e[3], t3);
-    v[7] = csub(e[3], t3);
-}
-
-template <int P, bool INV>
-__device__ __forceinline__ void dftp(double2* v)
-{
-    if constexpr (P == 8) dft8<INV>(v);
-    else if constexpr (P == 4) dft4<INV>(v);
-    else dft2<INV>(v);
-}
-
-// One Stockham autosort stage of radix P over NT transforms of length N in
-// LDS (transform t at lds + t*N).  Stage with span NS: butterfly j reads
-// j + r*N/P, twiddles by W_(NS*P)^(r*(j mod NS)), writes
-// (j/NS)*NS*P + (j mod NS) + r*NS.
-template <int N, int NT, int TPB, int P, int NS, bool INV>
-__device__ __forceinline__ void lds_stage(double2* lds, const double2* __restrict__ tw)
-{
-    constexpr int NB = NT * (N / P);
-    constexpr int PER = (NB + TPB - 1) / TPB;
-    constexpr int SCALE = 4096 / N;
-    double2 v[PER][P];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int t = threadIdx.x + u * TPB;
-        if ((NB % TPB == 0) || t < NB) {
-            const int tr = t / (N / P), j = t % (N / P);
-            const double2* base = lds + tr * N;
-#pragma unroll
-            for (int r = 0; r < P; ++r) v[u][r] = base[j + r * (N / P)];
-            if constexpr (NS > 1) {
-                const int jj = j % NS;
-#pragma unroll
-                for (int r = 1; r < P; ++r) {
-                    double2 w = tw[(r * jj * (N / (NS * P))) * SCALE];
-                    if (INV) w.y = -w.y;
-                    v[u][r] = cmul(v[u][r], w);
-                }
-            }
-            dftp<P, INV>(v[u]);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int t = threadIdx.x + u * TPB;
-        if ((NB % TPB == 0) || t < NB) {
-            const int tr = t / (N / P), j = t % (N / P);
-            double2* base = lds + tr * N;
-            const int o = (j / NS) * NS * P + (j % NS);
-#pragma unroll
-            for (int r = 0; r < P; ++r) base[o + r * NS] = v[u][r];
-        }
-    }
-    __syncthreads();
-}
-
-template <int N, int NT, int TPB, bool INV, int NS = 1>
-__device__ __forceinline__ void lds_fft(double2* lds, const double2* __restrict__ tw)
-{
-    if constexpr (NS < N) {
-        constexpr int REM = N / NS;
-        constexpr int P = REM >= 8 ? 8 : (REM >= 4 ? 4 : 2);
-        lds_stage<N, NT, TPB, P, NS, INV>(lds, tw);
-        lds_fft<N, NT, TPB, INV, NS * P>(lds, tw);
-    }
-}
-
-// ------------------------------------------------------------- FFT kernels
-
-// Forward column pass: NT consecutive columns c0.. of the R x C array,
-// R-point DFT down each column, then the W_L^(c*rho) twiddle.
-template <int R, int NT, int TPB>
-__global__ __launch_bounds__(TPB) void k_fwd_col(Params p)
-{
-    __shared__ double2 lds[NT * R];
-    const unsigned c0 = blockIdx.x * NT;
-    for (int t = threadIdx.x; t < NT * R; t += TPB) {
-        const int r = t / NT, b = t % NT;
-        lds[b * R + r] = p.z[(size_t)r * p.C + c0 + b];
-    }
-    __syncthreads();
-    lds_fft<R, NT, TPB, false>(lds, p.tw);
-    for (int t = threadIdx.x; t < NT * R; t += TPB) {
-        const int rho = t / NT, b = t % NT;
-        const unsigned c = c0 + b;
-        p.z[(size_t)rho * p.C + c] = cmul(lds[b * R + rho], twL(p, c * (unsigned)rho));
-    }
-}
-
-// Row pass: NT rows; forward C-point DFT, Z <- Z*(Z + 2i*X_hi), inverse.
-template <int C, int NT, int TPB>
-__global__ __launch_bounds__(TPB) void k_row(Params p)
-{
-    __shared__ double2 lds[NT * C];
-    const size_t base = (size_t)blockIdx.x * NT * C;
-    for (int t = threadIdx.x; t < NT * C; t += TPB) lds[t] = p.z[base + t];
-    __syncthreads();
-    lds_fft<C, NT, TPB, false>(lds, p.tw);
-    const double x2 = 2.0 * (double)(*p.xhi);
-    for (int t = threadIdx.x; t < NT * C; t += TPB) {
-        double2 X = lds[t];
-        lds[t] = cmul(X, make_double2(X.x, X.y + x2));
-    }
-    __syncthreads();
-    lds_fft<C, NT, TPB, true>(lds, p.tw);
-    for (int t = threadIdx.x; t < NT * C; t += TPB) p.z[base + t] = lds[t];
-}
-
-// Inverse column pass: conjugate twiddle, inverse R-point DFT down columns.
-template <int R, int NT, int TPB>
-__global__ __launch_bounds__(TPB) void k_inv_col(Params p)
-{
-    __shared__ double2 lds[NT * R];
-    const unsigned c0 = blockIdx.x * NT;
-    for (int t = threadIdx.x; t < NT * R; t += TPB) {
-        const int rho = t / NT, b = t % NT;
-        const unsigned c = c0 + b;
-        double2 w = twL(p, c * (unsigned)rho);
-        w.y = -w.y;
-        lds[b * R + rho] = cmul(p.z[(size_t)rho * p.C + c], w);
-    }
-    __syncthreads();
-    lds_fft<R, NT, TPB, true>(lds, p.tw);
-    for (int t = threadIdx.x; t < NT * R; t += TPB) {
-        const int r = t / NT, b = t % NT;
-        p.z[(size_t)r * p.C + c0 + b] = lds[b * R + r];
-    }
-}
-
-// ------------------------------------------------------------- carry
-
-// Folds one run's carry-in into its first digits.  c2in is the first fold's
+// Adjustment a run receives from its predecessor: c2in is the first fold's
 // carry arriving at the run's first bit (it already includes -k times the
-// previous run's T_lo carry); it is spread over balanced digits delta_j,
-// which enter the second fold as -k*delta_j; extra is the second fold's
-// carry-in at the same bit.  Both are normalised into this run's digits.
-template <int RUN>
-__device__ __forceinline__ void adjust_run(const Params& p, unsigned m0, long long* x, long long c2in, long long extra)
+// predecessor's T_lo carry); it is spread over balanced digits delta_j, which
+// enter the second fold as -k*delta_j; extra is the second fold's carry-in at
+// the same bit.  The sum is normalised into W balanced digits nu_j added to
+// the run's digits.  Sets status when W digits do not absorb it.
+template <int W>
+__device__ __forceinline__ void ripple(const Params& p, unsigned m0, long long c2in, long long extra, long long (&nu)[W])
 {
-    long long a[RUN];
     long long rem = c2in;
+    long long a[W];
 #pragma unroll
-    for (int j = 0; j < RUN; ++j) {
+    for (int j = 0; j < W; ++j) {
         long long c;
-        long long d = bal(rem, width(p, m0 + j), c);
+        const long long d = bal(rem, width(p, m0 + j), c);
         a[j] = -p.k * d;
         rem = c;
     }
-    if (rem != 0) atomicOr(p.status, 1);
     a[0] += extra;
     long long c = 0;
 #pragma unroll
-    for (int j = 0; j < RUN; ++j) {
+    for (int j = 0; j < W; ++j) {
         long long cc;
-        x[j] += bal(a[j] + c, width(p, m0 + j), cc);
+        nu[j] = bal(a[j] + c, width(p, m0 + j), cc);
         c = cc;
     }
-    if (c != 0) atomicOr(p.status, 2);
+    if (rem != 0 || c != 0) atomicOr(p.status, 1);
 }
 
 __device__ __forceinline__ double wave_max(double v)
@@ -337,6 +170,58 @@ __device__ __forceinline__ double wave_max(double v)
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
 }
+
+// ------------------------------------------------------------- FFT kernels
+
+template <int R, int NT, int TPB>
+__global__ __launch_bounds__(TPB) void k_fwd_col(Params p)
+{
+    __shared__ double2 lds[LdsLayout<R, NT>::kSlots];
+    const unsigned c0 = blockIdx.x * NT;
+    const double2* __restrict__ z = p.z;
+    dft<R, NT, TPB, false>(
+        lds, p.tw, [&](int tr, int idx) { return z[(size_t)idx * p.C + c0 + tr]; },
+        [&](int tr, int idx, double2 v) {
+            const unsigned c = c0 + tr;
+            p.z[(size_t)idx * p.C + c] = cmul(v, twL(p, c * (unsigned)idx));
+        });
+}
+
+template <int R, int NT, int TPB>
+__global__ __launch_bounds__(TPB) void k_inv_col(Params p)
+{
+    __shared__ double2 lds[LdsLayout<R, NT>::kSlots];
+    const unsigned c0 = blockIdx.x * NT;
+    const double2* __restrict__ z = p.z;
+    dft<R, NT, TPB, true>(
+        lds, p.tw,
+        [&](int tr, int idx) {
+            const unsigned c = c0 + tr;
+            return cmulc(z[(size_t)idx * p.C + c], twL(p, c * (unsigned)idx));
+        },
+        [&](int tr, int idx, double2 v) { p.z[(size_t)idx * p.C + c0 + tr] = v; });
+}
+
+// Row pass: NT rows; forward C-point DFT, Z <- Z*(Z + 2i*X_hi) folded into
+// the last forward stage, inverse C-point DFT.
+template <int C, int NT, int TPB>
+__global__ __launch_bounds__(TPB) void k_row(Params p)
+{
+    using Lay = LdsLayout<C, NT>;
+    __shared__ double2 lds[Lay::kSlots];
+    const size_t base = (size_t)blockIdx.x * NT * C;
+    const double2* __restrict__ z = p.z;
+    const double x2 = 2.0 * (double)(*p.xhi);
+    dft<C, NT, TPB, false, false, true>(
+        lds, p.tw, [&](int tr, int idx) { return z[base + (size_t)tr * C + idx]; },
+        [&](int tr, int idx, double2 X) { lds[Lay::at(tr, idx)] = cmul(X, make_double2(X.x, X.y + x2)); });
+    __syncthreads();
+    dft<C, NT, TPB, true, true>(
+        lds, p.tw, [&](int tr, int idx) { return lds[Lay::at(tr, idx)]; },
+        [&](int tr, int idx, double2 v) { p.z[base + (size_t)tr * C + idx] = v; });
+}
+
+// ------------------------------------------------------------- flat carry
 
 // Unweight, round (with max round-off), three carry passes per run of RUN
 // digits, in-block run hand-off, re-weight.  The first run of each block is
@@ -357,18 +242,17 @@ __global__ __launch_bounds__(TPB) void k_carry(Params p)
         const double2 v = cmul(p.z[m], inv_weight(p, m));
         const double tr = rint(v.x), hr = rint(v.y);
         emax = fmax(emax, fmax(fabs(v.x - tr), fabs(v.y - hr)));
-        const int b = width(p, m);
-        const long long d1 = bal((long long)tr + q1, b, q1);       // T_lo
-        const long long d2 = bal((long long)hr - k * d1 + q2, b, q2);  // S = T_hi - k*T_lo
-        x[j] = bal(-k * d2 + q3, b, q3);                            // R = S_hi - k*S_lo
+        x[j] = carry3(tr, hr, k, width(p, m), q1, q2, q3);
     }
     sq[0][tid] = q1;
     sq[1][tid] = q2;
     sq[2][tid] = q3;
     __syncthreads();
     if (tid > 0) {
-        const long long c1 = sq[0][tid - 1], c2 = sq[1][tid - 1], c3 = sq[2][tid - 1];
-        adjust_run<RUN>(p, m0, x, c2 - k * c1, c3);
+        long long nu[RUN];
+        ripple<RUN>(p, m0, sq[1][tid - 1] - k * sq[0][tid - 1], sq[2][tid - 1], nu);
+#pragma unroll
+        for (int j = 0; j < RUN; ++j) x[j] += nu[j];
     }
     if (tid == TPB - 1) {
         p.carries[3 * blockIdx.x + 0] = q1;
@@ -408,22 +292,141 @@ __global__ void k_fixup(Params p, unsigned nblocks)
         c2in = c[0];
         extra = c[1] + xh * xh;
     }
-    long long x[RUN];
+    long long nu[RUN];
+    ripple<RUN>(p, m0, c2in, extra, nu);
 #pragma unroll
     for (int j = 0; j < RUN; ++j) {
+        if (nu[j] == 0) continue;
         const unsigned m = m0 + j;
         const double2 v = cmul(p.z[m], inv_weight(p, m));
-        x[j] = llrint(v.x * (double)p.L);
-    }
-    adjust_run<RUN>(p, m0, x, c2in, extra);
-#pragma unroll
-    for (int j = 0; j < RUN; ++j) {
-        const unsigned m = m0 + j;
+        const double xd = rint(v.x * (double)p.L) + (double)nu[j];
         const double2 wt = fwd_weight(p, m);
-        const double xd = (double)x[j];
         p.z[m] = make_double2(xd * wt.x, xd * wt.y);
     }
     if (w == 0) *p.xhi = p.carries[3 * (nblocks - 1) + 2];
+}
+
+// ------------------------------------------------------------- fused outer pass
+
+__device__ __forceinline__ unsigned ld_acquire(const unsigned* f)
+{
+    return __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Block b owns columns [b*NT, b*NT+NT): for each row r its digits
+// r*C + b*NT .. +NT-1 form one run.  The predecessor of run (b, r) is
+// (b-1, r), or (nb-1, r-1) for b = 0; run (0, 0) takes the top rule.
+template <int R, int NT, int TPB>
+__global__ __launch_bounds__(TPB) void k_outer(Params p)
+{
+    using Lay = LdsLayout<R, NT>;
+    constexpr int RPT = R / TPB;  // runs per thread
+    static_assert(R % TPB == 0, "runs must split evenly over threads");
+    __shared__ double2 lds[Lay::kSlots];
+    __shared__ unsigned s_blk;
+    __shared__ double s_err[TPB / 64];
+
+    const unsigned nb = p.C / NT;
+    if (threadIdx.x == 0) s_blk = atomicAdd(p.ticket, 1u) % nb;
+    __syncthreads();
+    const unsigned b = s_blk;
+    const unsigned c0 = b * NT;
+    const long long k = p.k;
+
+    // 1. inverse column DFT into LDS (natural row order)
+    const double2* __restrict__ z = p.z;
+    dft<R, NT, TPB, true, false, true>(
+        lds, p.tw,
+        [&](int tr, int idx) {
+            const unsigned c = c0 + tr;
+            return cmulc(z[(size_t)idx * p.C + c], twL(p, c * (unsigned)idx));
+        },
+        [&](int tr, int idx, double2 v) { lds[Lay::at(tr, idx)] = v; });
+    __syncthreads();
+
+    // 2. own carries of each run
+    long long x[RPT][NT];
+    double emax = 0.0;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        const unsigned m0 = r * p.C + c0;
+        long long q1 = 0, q2 = 0, q3 = 0;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const unsigned m = m0 + j;
+            const double2 v = cmul(lds[Lay::at(j, r)], inv_weight(p, m));
+            const double tr = rint(v.x), hr = rint(v.y);
+            emax = fmax(emax, fmax(fabs(v.x - tr), fabs(v.y - hr)));
+            x[u][j] = carry3(tr, hr, k, width(p, m), q1, q2, q3);
+        }
+        long long* rc = p.runc + 3 * ((size_t)b * R + r);
+        rc[0] = q1;
+        rc[1] = q2;
+        rc[2] = q3;
+    }
+    // 3. publish, then wait for the predecessor block
+    __threadfence();
+    __syncthreads();
+    const unsigned pred = b == 0 ? nb - 1 : b - 1;
+    if (threadIdx.x == 0) {
+        const unsigned mine = __hip_atomic_fetch_add(p.flags + b, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) + 1;
+        while (ld_acquire(p.flags + pred) < mine) __builtin_amdgcn_s_sleep(2);
+    }
+    __syncthreads();
+    __threadfence();
+
+    // 4. fold the predecessor's carries into each run's first digits (only
+    //    block 0 reads X_hi: it is also the block that replaces it)
+    const long long xh = b == 0 ? *p.xhi : 0;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        const unsigned m0 = r * p.C + c0;
+        long long c2in, extra;
+        if (b == 0 && r == 0) {
+            const long long* rc = p.runc + 3 * ((size_t)(nb - 1) * R + (R - 1));
+            c2in = __hip_atomic_load(rc + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            extra = __hip_atomic_load(rc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + xh * xh;
+        } else {
+            const size_t src = b == 0 ? (size_t)(nb - 1) * R + (r - 1) : (size_t)(b - 1) * R + r;
+            const long long* rc = p.runc + 3 * src;
+            const long long c1 = __hip_atomic_load(rc + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const long long c2 = __hip_atomic_load(rc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            c2in = c2 - k * c1;
+            extra = __hip_atomic_load(rc + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        long long nu[NT];
+        ripple<NT>(p, m0, c2in, extra, nu);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const unsigned m = m0 + j;
+            const double2 w = fwd_weight(p, m);
+            const double xd = (double)(x[u][j] + nu[j]);
+            lds[Lay::at(j, r)] = make_double2(xd * w.x, xd * w.y);
+        }
+    }
+    if (b == 0 && threadIdx.x == 0) {
+        const long long* rc = p.runc + 3 * ((size_t)(nb - 1) * R + (R - 1));
+        *p.xhi = __hip_atomic_load(rc + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    emax = wave_max(emax);
+    if ((threadIdx.x & 63) == 0) s_err[threadIdx.x / 64] = emax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double e = s_err[0];
+#pragma unroll
+        for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
+        atomicMax(p.err, (unsigned long long)__double_as_longlong(e));
+    }
+
+    // 5. forward column DFT from LDS, twiddle, store
+    dft<R, NT, TPB, false, true, false>(
+        lds, p.tw, [&](int tr, int idx) { return lds[Lay::at(tr, idx)]; },
+        [&](int tr, int idx, double2 v) {
+            const unsigned c = c0 + tr;
+            p.z[(size_t)idx * p.C + c] = cmul(v, twL(p, c * (unsigned)idx));
+        });
 }
 
 __global__ void k_set_digits(Params p, const int* __restrict__ d)
@@ -453,6 +456,7 @@ __global__ void k_fill_u64(unsigned long long* a, int count)
 
 static constexpr int kRun = 16;
 static constexpr int kCarryTpb = 64;
+static constexpr int kOuterNT = 8;
 
 double max_bits_per_digit(uint32_t logL)
 {
@@ -460,6 +464,28 @@ double max_bits_per_digit(uint32_t logL)
     if (logL <= 17) return 18.0;
     if (logL <= 20) return 17.5;
     return 17.0;
+}
+
+static unsigned bitlen64(unsigned long long x)
+{
+    unsigned b = 0;
+    while (x) {
+        ++b;
+        x >>= 1;
+    }
+    return b;
+}
+
+// Digits a carry-in can reach (see ripple): bounds the carries from the
+// worst-case coefficient magnitude L*2^(2b+1).
+static int ripple_digits(const Plan& pl)
+{
+    const unsigned bmin = pl.n >> pl.logL;
+    const unsigned bmax = bmin + 1;
+    const unsigned kb = bitlen64(pl.k);
+    const unsigned c2bits = bmax + 4 + pl.logL + kb + 1;
+    const unsigned a3bits = c2bits + kb + 1;
+    return (int)((a3bits + 1 + bmin - 1) / bmin) + 1;
 }
 
 bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::string& why)
@@ -504,14 +530,16 @@ bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::strin
     out.k = k;
     out.L = L;
     out.logL = logL;
-    out.R = 1u << (logL / 2);
+    const uint32_t logR = std::max<uint32_t>(logL > 12 ? logL - 12 : 0, std::min<uint32_t>(9, logL / 2));
+    out.R = 1u << logR;
     out.C = L / out.R;
     out.s = (logL + 1) / 2;
+    out.fused = false;
     return true;
 }
 
-static Params make_params(const Plan& pl, double2* z, const DeviceTables* t, long long* carries, long long* xhi,
-                          unsigned long long* err, int* status)
+static Params make_params(const Plan& pl, double2* z, const DeviceTables* t, long long* carries, long long* runc,
+                          unsigned* flags, unsigned* ticket, long long* xhi, unsigned long long* err, int* status)
 {
     Params p;
     p.z = z;
@@ -525,6 +553,9 @@ static Params make_params(const Plan& pl, double2* z, const DeviceTables* t, lon
     p.om_lo = t->om_lo;
     p.om_hi = t->om_hi;
     p.carries = carries;
+    p.runc = runc;
+    p.flags = flags;
+    p.ticket = ticket;
     p.xhi = xhi;
     p.err = err;
     p.status = status;
@@ -540,34 +571,53 @@ static Params make_params(const Plan& pl, double2* z, const DeviceTables* t, lon
 }
 
 template <int R>
-static void launch_col(bool inverse, const Params& p, hipStream_t st)
+struct ColCfg {
+    static constexpr int NT = R <= 1024 ? 8 : 4;
+    static constexpr int TPB = (NT * R / 16) < 64 ? 64 : (NT * R / 16);
+};
+
+template <int C>
+struct RowCfg {
+    static constexpr int NT = (1024 / C) > 1 ? (1024 / C) : 1;
+    static constexpr int TPB = (NT * C / 16) < 64 ? 64 : (NT * C / 16);
+};
+
+enum class Kind { FwdCol, InvCol, Outer };
+
+template <int R>
+static void launch_col(Kind kind, const Params& p, hipStream_t st)
 {
-    constexpr int NT = (4096 / R) < 8 ? (4096 / R) : 8;
-    constexpr int TPB = NT * R >= 2048 ? 256 : (NT * R >= 1024 ? 128 : 64);
+    constexpr int NT = ColCfg<R>::NT, TPB = ColCfg<R>::TPB;
     dim3 grid(p.C / NT);
-    if (inverse) hipLaunchKernelGGL((k_inv_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
-    else hipLaunchKernelGGL((k_fwd_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
+    switch (kind) {
+    case Kind::FwdCol: hipLaunchKernelGGL((k_fwd_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p); break;
+    case Kind::InvCol: hipLaunchKernelGGL((k_inv_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p); break;
+    case Kind::Outer:
+        if constexpr (R >= 64 && R <= 1024 && NT == kOuterNT && R % TPB == 0)
+            hipLaunchKernelGGL((k_outer<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
+        else
+            throw std::runtime_error("no fused outer kernel for this column length");
+        break;
+    }
 }
 
 template <int C>
 static void launch_row(const Params& p, hipStream_t st)
 {
-    constexpr int NT = (1024 / C) > 1 ? (1024 / C) : 1;
-    constexpr int TPB = NT * C >= 2048 ? 256 : 128;
+    constexpr int NT = RowCfg<C>::NT, TPB = RowCfg<C>::TPB;
     dim3 grid(p.R / NT);
     hipLaunchKernelGGL((k_row<C, NT, TPB>), grid, dim3(TPB), 0, st, p);
 }
 
-static void dispatch_col(bool inverse, const Params& p, hipStream_t st)
+static void dispatch_col(Kind kind, const Params& p, hipStream_t st)
 {
     switch (p.R) {
-    case 32: launch_col<32>(inverse, p, st); break;
-    case 64: launch_col<64>(inverse, p, st); break;
-    case 128: launch_col<128>(inverse, p, st); break;
-    case 256: launch_col<256>(inverse, p, st); break;
-    case 512: launch_col<512>(inverse, p, st); break;
-    case 1024: launch_col<1024>(inverse, p, st); break;
-    case 2048: launch_col<2048>(inverse, p, st); break;
+    case 32: launch_col<32>(kind, p, st); break;
+    case 64: launch_col<64>(kind, p, st); break;
+    case 128: launch_col<128>(kind, p, st); break;
+    case 256: launch_col<256>(kind, p, st); break;
+    case 512: launch_col<512>(kind, p, st); break;
+    case 1024: launch_col<1024>(kind, p, st); break;
     default: throw std::runtime_error("unsupported column length");
     }
 }
@@ -584,6 +634,30 @@ static void dispatch_row(const Params& p, hipStream_t st)
     case 2048: launch_row<2048>(p, st); break;
     case 4096: launch_row<4096>(p, st); break;
     default: throw std::runtime_error("unsupported row length");
+    }
+}
+
+template <int R>
+static int outer_max_blocks_per_cu()
+{
+    constexpr int NT = ColCfg<R>::NT, TPB = ColCfg<R>::TPB;
+    if constexpr (R >= 64 && R <= 1024 && NT == kOuterNT && R % TPB == 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_outer<R, NT, TPB>, TPB, 0) != hipSuccess) return 0;
+        return nb;
+    }
+    return 0;
+}
+
+static int outer_blocks_per_cu(uint32_t R)
+{
+    switch (R) {
+    case 64: return outer_max_blocks_per_cu<64>();
+    case 128: return outer_max_blocks_per_cu<128>();
+    case 256: return outer_max_blocks_per_cu<256>();
+    case 512: return outer_max_blocks_per_cu<512>();
+    case 1024: return outer_max_blocks_per_cu<1024>();
+    default: return 0;
     }
 }
 
@@ -627,7 +701,7 @@ static void host_tables(const Plan& pl, std::vector<double2>& tw, std::vector<do
     }
 }
 
-Engine::Engine(const Plan& p, int device) : plan_(p), device_(device)
+Engine::Engine(const Plan& p, int device, bool allow_fused) : plan_(p), device_(device)
 {
     check(hipSetDevice(device), "hipSetDevice");
     check(hipStreamCreate(&stream_), "hipStreamCreate");
@@ -636,14 +710,29 @@ Engine::Engine(const Plan& p, int device) : plan_(p), device_(device)
     host_tables(p, tw, twl_lo, twl_hi, om_lo, om_hi, pw_lo, pw_hi, ipw_lo, ipw_hi);
     carry_blocks_ = p.L / (kRun * kCarryTpb);
 
+    // The fused pass needs: a kernel for this column length, runs (NT digits)
+    // long enough for any carry-in to settle, and every block resident at
+    // once (blocks spin on their predecessor's flag).
+    if (allow_fused && ripple_digits(p) <= kOuterNT) {
+        int bpc = outer_blocks_per_cu(p.R);
+        int cus = 0;
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
+        const unsigned nb = p.C / kOuterNT;
+        if (bpc > 0 && (long long)bpc * cus >= (long long)nb) plan_.fused = true;
+    }
+    const unsigned outer_blocks = p.C / kOuterNT;
+
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t off = 0;
     const size_t o_z = off; off += al(sizeof(double2) * p.L);
     const size_t o_car = off; off += al(sizeof(long long) * 3 * carry_blocks_);
+    const size_t o_runc = off; off += al(plan_.fused ? sizeof(long long) * 3 * (size_t)p.R * outer_blocks : 0);
+    const size_t o_flags = off; off += al(sizeof(unsigned) * (outer_blocks + 1));
     const size_t o_xhi = off; off += al(sizeof(long long));
     const size_t o_err = off; off += al(sizeof(unsigned long long) * kGraphChunk);
     const size_t o_st = off; off += al(sizeof(int));
     const size_t o_dig = off; off += al(sizeof(int32_t) * p.L);
+    const size_t zero_end = off;
     const size_t o_tw = off; off += al(sizeof(double2) * tw.size());
     const size_t o_tlo = off; off += al(sizeof(double2) * twl_lo.size());
     const size_t o_thi = off; off += al(sizeof(double2) * twl_hi.size());
@@ -658,6 +747,9 @@ Engine::Engine(const Plan& p, int device) : plan_(p), device_(device)
     char* b = static_cast<char*>(mem_);
     z_ = reinterpret_cast<double2*>(b + o_z);
     carries_ = reinterpret_cast<long long*>(b + o_car);
+    runc_ = reinterpret_cast<long long*>(b + o_runc);
+    flags_ = reinterpret_cast<unsigned*>(b + o_flags);
+    ticket_ = flags_ + outer_blocks;
     xhi_ = reinterpret_cast<long long*>(b + o_xhi);
     err_ = reinterpret_cast<unsigned long long*>(b + o_err);
     status_ = reinterpret_cast<int*>(b + o_st);
@@ -684,7 +776,7 @@ Engine::Engine(const Plan& p, int device) : plan_(p), device_(device)
     up(tabs_->pw_hi, pw_hi.data(), sizeof(double) * pw_hi.size());
     up(tabs_->ipw_lo, ipw_lo.data(), sizeof(double) * ipw_lo.size());
     up(tabs_->ipw_hi, ipw_hi.data(), sizeof(double) * ipw_hi.size());
-    check(hipMemsetAsync(mem_, 0, o_tw, stream_), "hipMemsetAsync");
+    check(hipMemsetAsync(mem_, 0, zero_end, stream_), "hipMemsetAsync");
     check(hipStreamSynchronize(stream_), "init sync");
     err_host_.resize(kGraphChunk);
 }
@@ -698,6 +790,11 @@ Engine::~Engine()
     delete tabs_;
 }
 
+Params Engine::params(uint64_t slot) const
+{
+    return make_params(plan_, z_, tabs_, carries_, runc_, flags_, ticket_, xhi_, err_ + slot, status_);
+}
+
 void Engine::upload(const std::vector<int32_t>& digits, int64_t xhi)
 {
     check(hipSetDevice(device_), "hipSetDevice");
@@ -706,7 +803,7 @@ void Engine::upload(const std::vector<int32_t>& digits, int64_t xhi)
     check(hipMemcpyAsync(digits_dev_, digits.data(), sizeof(int32_t) * plan_.L, hipMemcpyHostToDevice, stream_),
           "upload digits");
     check(hipMemcpyAsync(xhi_, &xh, sizeof(long long), hipMemcpyHostToDevice, stream_), "upload xhi");
-    Params p = make_params(plan_, z_, tabs_, carries_, xhi_, err_, status_);
+    Params p = params(0);
     hipLaunchKernelGGL(k_set_digits, dim3((plan_.L + 255) / 256), dim3(256), 0, stream_, p, digits_dev_);
     check(hipGetLastError(), "k_set_digits");
     check(hipStreamSynchronize(stream_), "upload sync");
@@ -715,7 +812,7 @@ void Engine::upload(const std::vector<int32_t>& digits, int64_t xhi)
 void Engine::download(std::vector<int32_t>& digits, int64_t& xhi)
 {
     check(hipSetDevice(device_), "hipSetDevice");
-    Params p = make_params(plan_, z_, tabs_, carries_, xhi_, err_, status_);
+    Params p = params(0);
     hipLaunchKernelGGL(k_get_digits, dim3((plan_.L + 255) / 256), dim3(256), 0, stream_, p, digits_dev_);
     check(hipGetLastError(), "k_get_digits");
     digits.resize(plan_.L);
@@ -727,14 +824,30 @@ void Engine::download(std::vector<int32_t>& digits, int64_t& xhi)
     xhi = xh;
 }
 
-void Engine::enqueue_one(uint64_t slot)
+// g squarings from the digit state back to the digit state; iteration i's
+// round-off lands in err slot i.
+void Engine::enqueue_chunk(int g)
 {
-    Params p = make_params(plan_, z_, tabs_, carries_, xhi_, err_ + slot, status_);
-    dispatch_col(false, p, stream_);
-    dispatch_row(p, stream_);
-    dispatch_col(true, p, stream_);
-    hipLaunchKernelGGL((k_carry<kRun, kCarryTpb>), dim3(carry_blocks_), dim3(kCarryTpb), 0, stream_, p);
-    hipLaunchKernelGGL((k_fixup<kRun, kCarryTpb>), dim3((carry_blocks_ + 63) / 64), dim3(64), 0, stream_, p,
+    const Params p0 = params(0);
+    dispatch_col(Kind::FwdCol, p0, stream_);
+    dispatch_row(p0, stream_);
+    for (int i = 0; i + 1 < g; ++i) {
+        const Params p = params((uint64_t)i);
+        if (plan_.fused) {
+            dispatch_col(Kind::Outer, p, stream_);
+        } else {
+            dispatch_col(Kind::InvCol, p, stream_);
+            hipLaunchKernelGGL((k_carry<kRun, kCarryTpb>), dim3(carry_blocks_), dim3(kCarryTpb), 0, stream_, p);
+            hipLaunchKernelGGL((k_fixup<kRun, kCarryTpb>), dim3((carry_blocks_ + 63) / 64), dim3(64), 0, stream_, p,
+                               carry_blocks_);
+            dispatch_col(Kind::FwdCol, p, stream_);
+        }
+        dispatch_row(p, stream_);
+    }
+    const Params pl = params((uint64_t)(g - 1));
+    dispatch_col(Kind::InvCol, pl, stream_);
+    hipLaunchKernelGGL((k_carry<kRun, kCarryTpb>), dim3(carry_blocks_), dim3(kCarryTpb), 0, stream_, pl);
+    hipLaunchKernelGGL((k_fixup<kRun, kCarryTpb>), dim3((carry_blocks_ + 63) / 64), dim3(64), 0, stream_, pl,
                        carry_blocks_);
 }
 
@@ -750,7 +863,7 @@ void Engine::build_graph(int iters)
     }
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kGraphChunk), 0, stream_, err_, kGraphChunk);
-    for (int i = 0; i < iters; ++i) enqueue_one((uint64_t)i);
+    enqueue_chunk(iters);
     check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
     check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
     graph_iters_ = iters;
@@ -768,7 +881,7 @@ double Engine::square(uint64_t iters, double* err_trace)
             check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
         } else {
             hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kGraphChunk), 0, stream_, err_, kGraphChunk);
-            for (int i = 0; i < g; ++i) enqueue_one((uint64_t)i);
+            enqueue_chunk(g);
             check(hipGetLastError(), "square launch");
         }
         check(hipMemcpyAsync(err_host_.data(), err_, sizeof(unsigned long long) * g, hipMemcpyDeviceToHost, stream_),
@@ -785,40 +898,45 @@ double Engine::square(uint64_t iters, double* err_trace)
     return emax;
 }
 
-void Engine::kernel_times(int reps, double* ms5)
+void Engine::kernel_times(int reps, double* ms)
 {
     check(hipSetDevice(device_), "hipSetDevice");
     std::vector<int32_t> saved;
     int64_t saved_xhi = 0;
     download(saved, saved_xhi);
-    Params p = make_params(plan_, z_, tabs_, carries_, xhi_, err_, status_);
+    const Params p = params(0);
     hipEvent_t a, b;
     check(hipEventCreate(&a), "hipEventCreate");
     check(hipEventCreate(&b), "hipEventCreate");
     for (int kind = 0; kind < kNumKernels; ++kind) {
-        // one untimed launch, then reps timed launches
+        if (kind == 5 && !plan_.fused) {
+            ms[kind] = 0.0;
+            continue;
+        }
         for (int pass = 0; pass < 2; ++pass) {
             const int cnt = pass == 0 ? 1 : reps;
             if (pass == 1) check(hipEventRecord(a, stream_), "hipEventRecord");
             for (int i = 0; i < cnt; ++i) {
                 switch (kind) {
-                case 0: dispatch_col(false, p, stream_); break;
+                case 0: dispatch_col(Kind::FwdCol, p, stream_); break;
                 case 1: dispatch_row(p, stream_); break;
-                case 2: dispatch_col(true, p, stream_); break;
+                case 2: dispatch_col(Kind::InvCol, p, stream_); break;
                 case 3:
                     hipLaunchKernelGGL((k_carry<kRun, kCarryTpb>), dim3(carry_blocks_), dim3(kCarryTpb), 0, stream_, p);
                     break;
-                default:
+                case 4:
                     hipLaunchKernelGGL((k_fixup<kRun, kCarryTpb>), dim3((carry_blocks_ + 63) / 64), dim3(64), 0,
                                        stream_, p, carry_blocks_);
+                    break;
+                default: dispatch_col(Kind::Outer, p, stream_); break;
                 }
             }
             if (pass == 1) check(hipEventRecord(b, stream_), "hipEventRecord");
         }
         check(hipEventSynchronize(b), "hipEventSynchronize");
-        float ms = 0;
-        check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-        ms5[kind] = (double)ms / reps;
+        float t = 0;
+        check(hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
+        ms[kind] = (double)t / reps;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

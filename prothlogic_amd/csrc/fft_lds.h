@@ -1,0 +1,247 @@
+// fft_lds.h -- device building blocks: radix-16/8/4/2 Stockham DFT stages
+// over NT interleaved transforms of length N staged in LDS.
+//
+// Thread t of a TPB-thread workgroup owns transform tr = t % NT and the
+// butterflies j = t/NT + u*(TPB/NT) of every stage, so a wavefront touches
+// NT adjacent transforms at each index: global accesses of a column tile are
+// NT*16 contiguous bytes, and LDS accesses hit consecutive 16-byte slots.
+//
+// LDS layout: element idx of transform tr lives at NT*(idx + idx/16) + tr.
+// The one-slot pad every 16 elements keeps the radix-16 write pattern
+// idx = 16*j + r conflict-free (lanes with successive j land NT slots apart).
+//
+// A transform is run as: first stage reading from a caller functor (global
+// memory or LDS), middle stages LDS -> LDS, last stage writing to a caller
+// functor (global memory or LDS).  Stage with span NS and radix P:
+// butterfly j reads j + r*N/P, twiddles by W_(NS*P)^(r*(j mod NS)), applies
+// the P-point DFT, writes (j/NS)*NS*P + (j mod NS) + r*NS (natural order out).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pleng {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b)  // a * conj(b)
+{
+    return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -a.x * b.y));
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+template <bool INV>
+__device__ __forceinline__ double2 mul_mi(double2 a)  // forward: -i*a, inverse: +i*a
+{
+    return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(double2* v)
+{
+    double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(double2& a0, double2& a1, double2& a2, double2& a3)
+{
+    double2 s0 = cadd(a0, a2), s1 = csub(a0, a2);
+    double2 s2 = cadd(a1, a3), s3 = csub(a1, a3);
+    double2 d = mul_mi<INV>(s3);
+    a0 = cadd(s0, s2);
+    a1 = cadd(s1, d);
+    a2 = csub(s0, s2);
+    a3 = csub(s1, d);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(double2* v)
+{
+    dft4<INV>(v[0], v[1], v[2], v[3]);
+}
+
+// multiply by W8^q = exp(-+ i*pi*q/4), q = 1, 2, 3
+template <bool INV, int Q>
+__device__ __forceinline__ double2 w8(double2 a)
+{
+    constexpr double h = 0.70710678118654752440;
+    constexpr double sg = INV ? -1.0 : 1.0;
+    if constexpr (Q == 1) return make_double2(h * (a.x + sg * a.y), h * (a.y - sg * a.x));
+    else if constexpr (Q == 2) return mul_mi<INV>(a);
+    else return make_double2(h * (-a.x + sg * a.y), h * (-a.y - sg * a.x));
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft8(double2* v)
+{
+    double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    double2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    o1 = w8<INV, 1>(o1);
+    o2 = w8<INV, 2>(o2);
+    o3 = w8<INV, 3>(o3);
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+}
+
+// 16-point DFT as 4 x 4: inner DFT-4 over n1 (stride 4), twiddle
+// W16^(n2*k1), outer DFT-4 over n2; X[k1 + 4*k2] lands in v[k1 + 4*k2].
+template <bool INV>
+__device__ __forceinline__ void dft16(double2* v)
+{
+    constexpr double c1 = 0.92387953251128675613;  // cos(pi/8)
+    constexpr double s1 = 0.38268343236508977173;  // sin(pi/8)
+    constexpr double sg = INV ? -1.0 : 1.0;
+    double2 y[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        double2 a0 = v[n2], a1 = v[4 + n2], a2 = v[8 + n2], a3 = v[12 + n2];
+        dft4<INV>(a0, a1, a2, a3);
+        y[4 * n2 + 0] = a0;
+        y[4 * n2 + 1] = a1;
+        y[4 * n2 + 2] = a2;
+        y[4 * n2 + 3] = a3;
+    }
+    // twiddles W16^(n2*k1) for n2, k1 in 1..3: exponents 1,2,3 / 2,4,6 / 3,6,9
+    const double2 w1 = make_double2(c1, -sg * s1);
+    const double2 w3 = make_double2(s1, -sg * c1);
+    const double2 w9 = make_double2(-c1, sg * s1);  // W16^9 = -W16^1
+    y[5] = cmul(y[5], w1);
+    y[6] = w8<INV, 1>(y[6]);
+    y[7] = cmul(y[7], w3);
+    y[9] = w8<INV, 1>(y[9]);
+    y[10] = w8<INV, 2>(y[10]);
+    y[11] = w8<INV, 3>(y[11]);
+    y[13] = cmul(y[13], w3);
+    y[14] = w8<INV, 3>(y[14]);
+    y[15] = cmul(y[15], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        double2 a0 = y[k1], a1 = y[4 + k1], a2 = y[8 + k1], a3 = y[12 + k1];
+        dft4<INV>(a0, a1, a2, a3);
+        v[k1] = a0;
+        v[k1 + 4] = a1;
+        v[k1 + 8] = a2;
+        v[k1 + 12] = a3;
+    }
+}
+
+template <int P, bool INV>
+__device__ __forceinline__ void dftp(double2* v)
+{
+    if constexpr (P == 16) dft16<INV>(v);
+    else if constexpr (P == 8) dft8<INV>(v);
+    else if constexpr (P == 4) dft4<INV>(v);
+    else dft2<INV>(v);
+}
+
+template <int N>
+struct Radix {
+    // radix of the stage whose span is NS
+    static constexpr int of(int NS)
+    {
+        return (N / NS) >= 16 ? 16 : ((N / NS) >= 8 ? 8 : ((N / NS) >= 4 ? 4 : 2));
+    }
+};
+
+template <int N, int NT>
+struct LdsLayout {
+    static constexpr int kSlots = NT * (N + N / 16);
+    __device__ static __forceinline__ int at(int tr, int idx) { return NT * (idx + (idx >> 4)) + tr; }
+};
+
+// One stage.  ld(tr, idx) -> double2, st(tr, idx, v).  When both go to the
+// same LDS buffer the caller passes SYNC_BETWEEN = true so every read of the
+// stage completes before any write.
+template <int N, int NT, int TPB, int P, int NS, bool INV, bool SYNC_BETWEEN, class Ld, class St>
+__device__ __forceinline__ void stage(const double2* __restrict__ tw, Ld ld, St st)
+{
+    constexpr int G = TPB / NT;       // butterfly groups per transform
+    constexpr int NBT = N / P;        // butterflies per transform
+    constexpr int PER = NBT >= G ? NBT / G : 1;
+    constexpr int SCALE = 4096 / N;   // tw is the 4096-point table
+    static_assert(TPB % NT == 0, "TPB must be a multiple of NT");
+    static_assert(NBT < G || NBT % G == 0, "butterflies must split evenly");
+    const int tr = threadIdx.x % NT;
+    const int g = threadIdx.x / NT;
+    double2 v[PER][P];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int j = g + u * G;
+        if (NBT >= G || j < NBT) {
+#pragma unroll
+            for (int r = 0; r < P; ++r) v[u][r] = ld(tr, j + r * NBT);
+            if constexpr (NS > 1) {
+                const int jj = j % NS;
+#pragma unroll
+                for (int r = 1; r < P; ++r) {
+                    double2 w = tw[(r * jj * (N / (NS * P))) * SCALE];
+                    if (INV) w.y = -w.y;
+                    v[u][r] = cmul(v[u][r], w);
+                }
+            }
+            dftp<P, INV>(v[u]);
+        }
+    }
+    if constexpr (SYNC_BETWEEN) __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int j = g + u * G;
+        if (NBT >= G || j < NBT) {
+            const int o = (j / NS) * NS * P + (j % NS);
+#pragma unroll
+            for (int r = 0; r < P; ++r) st(tr, o + r * NS, v[u][r]);
+        }
+    }
+}
+
+// Full N-point DFT of NT transforms: first stage ld_in -> LDS, middle
+// stages in LDS, last stage LDS -> st_out.  lds must hold
+// LdsLayout<N,NT>::kSlots double2.  IN_LDS: ld_in reads this same LDS
+// buffer; OUT_LDS: st_out writes it (each then needs a barrier between the
+// stage's reads and writes).  The caller provides the barrier before the
+// call when LDS was in use; the call ends without a trailing barrier.
+template <int N, int NT, int TPB, bool INV, bool IN_LDS, bool OUT_LDS, int NS, class LdIn, class StOut>
+__device__ __forceinline__ void dft_rest(double2* lds, const double2* __restrict__ tw, LdIn ld_in, StOut st_out)
+{
+    using Lay = LdsLayout<N, NT>;
+    constexpr int P = Radix<N>::of(NS);
+    if constexpr (NS * P >= N) {
+        // last stage
+        if constexpr (NS == 1) {
+            stage<N, NT, TPB, P, NS, INV, IN_LDS && OUT_LDS>(tw, ld_in, st_out);
+        } else {
+            stage<N, NT, TPB, P, NS, INV, OUT_LDS>(
+                tw, [&](int tr, int idx) { return lds[Lay::at(tr, idx)]; }, st_out);
+        }
+    } else {
+        if constexpr (NS == 1) {
+            stage<N, NT, TPB, P, NS, INV, IN_LDS>(
+                tw, ld_in, [&](int tr, int idx, double2 v) { lds[Lay::at(tr, idx)] = v; });
+        } else {
+            stage<N, NT, TPB, P, NS, INV, true>(
+                tw, [&](int tr, int idx) { return lds[Lay::at(tr, idx)]; },
+                [&](int tr, int idx, double2 v) { lds[Lay::at(tr, idx)] = v; });
+        }
+        __syncthreads();
+        dft_rest<N, NT, TPB, INV, IN_LDS, OUT_LDS, NS * P>(lds, tw, ld_in, st_out);
+    }
+}
+
+template <int N, int NT, int TPB, bool INV, bool IN_LDS = false, bool OUT_LDS = false, class LdIn, class StOut>
+__device__ __forceinline__ void dft(double2* lds, const double2* __restrict__ tw, LdIn ld_in, StOut st_out)
+{
+    dft_rest<N, NT, TPB, INV, IN_LDS, OUT_LDS, 1>(lds, tw, ld_in, st_out);
+}
+
+}  // namespace pleng

@@ -41,6 +41,26 @@ def test_square_exact_vs_python_int(gpu, k, n, L):
             assert 0 <= e < 0.25
 
 
+@pytest.mark.parametrize("fused", ["1", "0"])
+@pytest.mark.parametrize("k,n,L", [(3, 20000, 2048), (1048573, 60000, 4096), (11887, 300007, 16384),
+                                   (433917, 2097152, 131072), (65535, 3000017, 262144)])
+def test_fused_and_unfused_paths(gpu, monkeypatch, fused, k, n, L):
+    monkeypatch.setenv("PL_FUSED", fused)
+    rnd = random.Random(n)
+    N = k * (1 << n) + 1
+    x = rnd.randrange(N)
+    iters = 130 if n < 10 ** 6 else 70      # > one 64-squaring graph chunk plus a partial one
+    with gpu.Context(k, n, L) as c:
+        if fused == "0":
+            assert c.info.passes == 4
+        elif n / L >= 12:
+            assert c.info.passes == 2, "fused outer pass expected for this length"
+        c.set_residue(x)
+        e, tr = c.square(iters, trace=True)
+        assert 0 < e < 0.3 and len(tr) == iters
+        assert c.get_residue() == G.powm(x, 1 << iters, N)
+
+
 @pytest.mark.parametrize("k,n,L", [(3, 20000, 2048), (1048573, 60000, 4096), (11887, 300007, 16384)])
 def test_square_chain_vs_powm(gpu, k, n, L):
     rnd = random.Random(7)
