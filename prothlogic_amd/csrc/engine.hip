@@ -371,7 +371,17 @@ __global__ __launch_bounds__(TPB) void k_outer(Params p)
     const unsigned pred = b == 0 ? nb - 1 : b - 1;
     if (threadIdx.x == 0) {
         const unsigned mine = __hip_atomic_fetch_add(p.flags + b, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) + 1;
-        while (ld_acquire(p.flags + pred) < mine) __builtin_amdgcn_s_sleep(2);
+        // bounded wait: every block publishes before it waits, so the bound
+        // is never reached in a healthy launch; if it is, the status word
+        // reports it and the kernel still exits
+        unsigned spins = 0;
+        while (ld_acquire(p.flags + pred) < mine) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > (1u << 22)) {
+                atomicOr(p.status, 4);
+                break;
+            }
+        }
     }
     __syncthreads();
     __threadfence();
