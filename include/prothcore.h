@@ -43,8 +43,8 @@ typedef struct pl_info {
     uint32_t cols;         /* C: contiguous DFT length                      */
     double bits_per_digit; /* n / L                                         */
     uint64_t device_bytes; /* HBM held by the context                       */
-    uint32_t passes;       /* full-array kernel passes per steady-state squaring:
-                              2 with the fused outer pass, 4 without        */
+    uint32_t passes;       /* kernels per steady-state squaring: 3 with the
+                              split outer pass, 5 without                    */
 } pl_info;
 
 /* Thread-local description of the last error (never NULL). */
@@ -99,11 +99,12 @@ int pl_fft_square_karatsuba(pl_ctx* ctx, uint64_t* v, size_t n_limbs, unsigned n
 
 /* Diagnostics (no counterpart in the reference): average duration in ms of
  * each kernel of one squaring, timed with HIP events on the context's own
- * stream over `reps` launches, into ms[0..5]: ms[0] forward column pass,
+ * stream over `reps` launches, into ms[0..6]: ms[0] forward column pass,
  * ms[1] row pass (forward DFT, pointwise square, inverse DFT), ms[2] inverse
- * column pass, ms[3] flat carry/fold, ms[4] block fix-up, ms[5] fused outer
- * pass (inverse column DFT + carry/fold + forward column DFT; 0 when the
- * context does not use it).  The residue is preserved.
+ * column pass, ms[3] flat carry/fold, ms[4] block fix-up, ms[5] inverse
+ * column DFT + carry/fold, ms[6] carry-in + forward column DFT (ms[5] and
+ * ms[6] are 0 when the context does not use the split outer pass).  The
+ * residue is preserved.
  * *traffic_bytes (nullable) gets the algorithmic HBM bytes one launch of
  * each full-array kernel moves (read + write of the L complex points). */
 int pl_kernel_times(pl_ctx* ctx, int reps, double* ms, uint64_t* traffic_bytes);

@@ -39,10 +39,13 @@ def _run(cmd, cwd=None):
     subprocess.run([str(c) for c in cmd], check=True, cwd=cwd)
 
 
-def build_core(verbose: bool = True) -> Path:
-    out = PKG / "libprothcore.so"
+def build_core(defines: tuple = (), out: Path | None = None) -> Path:
+    """hipcc -> libprothcore.so.  `defines` (e.g. ("PL_MAX_RADIX=8",)) and
+    `out` build a variant library for A/B measurements."""
+    out = Path(out) if out else PKG / "libprothcore.so"
     tmp = out.with_suffix(".so.tmp")
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp, *CORE_SRCS]
+    cmd = [HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-shared", f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp,
+           *CORE_SRCS]
     _run(cmd)
     os.replace(tmp, out)
     return out

@@ -173,7 +173,7 @@ int pl_ctx_info(const pl_ctx* ctx, pl_info* out)
     out->cols = ctx->plan.C;
     out->bits_per_digit = (double)ctx->plan.n / (double)ctx->plan.L;
     out->device_bytes = ctx->eng->device_bytes();
-    out->passes = ctx->plan.fused ? 2 : 4;
+    out->passes = ctx->plan.fused ? 3 : 5;
     return PL_OK;
 }
 

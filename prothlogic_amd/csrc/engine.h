@@ -17,7 +17,7 @@ struct Plan {
     uint64_t k = 0;
     uint32_t L = 0, logL = 0, R = 0, C = 0;
     uint32_t s = 0;      // log2 of the low half of the two-level tables
-    bool fused = false;  // steady state uses the fused outer pass (2 kernels per squaring)
+    bool fused = false;  // steady state uses the split outer pass (3 kernels per squaring)
 };
 
 // Rules for L: smallest power of two in [2^10, 2^22] whose bits per digit
@@ -54,15 +54,18 @@ class Engine {
     // Average duration (ms) of each kernel, timed with HIP events on this
     // engine's stream over `reps` launches each: [0] forward column pass,
     // [1] row pass (DFT, square, inverse DFT), [2] inverse column pass,
-    // [3] flat carry, [4] block fix-up, [5] fused outer pass (0 when the
-    // plan does not use it).  The residue is saved before and restored after.
+    // [3] flat carry, [4] block fix-up, [5] inverse column DFT + carry
+    // (split outer pass), [6] carry-in + forward column DFT (split outer
+    // pass); [5] and [6] are 0 when the plan does not use them.  The residue
+    // is saved before and restored after.
     void kernel_times(int reps, double* ms);
-    static constexpr int kNumKernels = 6;
+    static constexpr int kNumKernels = 7;
 
     static constexpr int kGraphChunk = 64;
 
   private:
     Params params(uint64_t slot) const;
+    void carry(const Params& p, bool do_carry, bool do_fixup);
     void enqueue_chunk(int g);
     void build_graph(int iters);
 
@@ -74,14 +77,15 @@ class Engine {
     double2* z_ = nullptr;
     long long* carries_ = nullptr;
     long long* runc_ = nullptr;
-    unsigned* flags_ = nullptr;
-    unsigned* ticket_ = nullptr;
+    int* rdig_ = nullptr;
     long long* xhi_ = nullptr;
     unsigned long long* err_ = nullptr;  // kGraphChunk slots (bit patterns of non-negative doubles)
     int* status_ = nullptr;
     int32_t* digits_dev_ = nullptr;
     DeviceTables* tabs_ = nullptr;
     uint32_t carry_blocks_ = 0;
+    int run_ = 16;         // digits per thread in the flat carry kernel
+    int carry_tpb_ = 64;   // threads per flat carry block
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
     int graph_iters_ = 0;

@@ -29,9 +29,11 @@
 // R-point DFTs down the columns with the W_L^(c*rho) twiddle, then C-point
 // DFTs along rows; the spectrum stays in that transposed order.  One squaring
 // is two kernels in steady state:
-//   k_outer : inverse column DFT -> unweight/round/carry/fold -> weight ->
-//             forward column DFT (block of NT columns, runs of NT digits)
-//   k_row   : forward row DFT -> Z*(Z + 2i*X_hi) -> inverse row DFT
+//   k_inv_carry  : inverse column DFT -> unweight/round -> own carries of
+//                  each run of NT digits -> int32 digits + run carry-outs
+//   k_adjust_fwd : predecessor run's carries folded in -> weight -> forward
+//                  column DFT
+//   k_row        : forward row DFT -> Z*(Z + 2i*X_hi) -> inverse row DFT
 // k_fwd_col / k_inv_col / k_carry / k_fixup are the unfused pieces used to
 // enter and leave the steady state (and for lengths the fused path does not
 // cover).
@@ -63,9 +65,8 @@ struct Params {
     const double2* om_lo;   // exp(i*pi*j/(2L))
     const double2* om_hi;   // exp(i*pi*j*2^s/(2L))
     long long* carries;     // flat carry kernel: 3 per block (own carry-outs of its last run)
-    long long* runc;        // outer kernel: 3 per run (block-major, then row)
-    unsigned* flags;        // outer kernel: per-block publish counters
-    unsigned* ticket;       // outer kernel: logical block ticket counter
+    long long* runc;        // split outer pass: 3 carry-outs per run (block-major, then row)
+    int* dig;               // split outer pass: NT int32 digits per run (same order)
     long long* xhi;
     unsigned long long* err;
     int* status;
@@ -108,6 +109,18 @@ __device__ __forceinline__ double2 fwd_weight(const Params& p, unsigned m)
     double2 w = omega(p, m);
     return make_double2(w.x * a, w.y * a);
 }
+__device__ __forceinline__ double2 fwd_weight_f(const Params& p, unsigned m, unsigned f)
+{
+    double a = p.pw_hi[f >> p.s] * p.pw_lo[f & p.smask];
+    double2 w = omega(p, m);
+    return make_double2(w.x * a, w.y * a);
+}
+__device__ __forceinline__ double2 inv_weight_f(const Params& p, unsigned m, unsigned f)
+{
+    double ia = p.ipw_hi[f >> p.s] * p.ipw_lo[f & p.smask];
+    double2 w = omega(p, m);
+    return make_double2(w.x * ia, -w.y * ia);
+}
 // conj(w^m) / (L * a_m)
 __device__ __forceinline__ double2 inv_weight(const Params& p, unsigned m)
 {
@@ -125,14 +138,39 @@ __device__ __forceinline__ long long bal(long long v, int b, long long& carry)
     return v - q * (1ll << b);
 }
 
-// One digit through the three carry passes: T_lo, S = T_hi - k*T_lo,
-// R = S_hi - k*S_lo.  Returns the digit of R; advances the carries.
-__device__ __forceinline__ long long carry3(double tr, double hr, long long k, int b, long long& q1, long long& q2,
-                                           long long& q3)
+// Exact balanced split in FP64 for integers below 2^53: v = d + q*2^b.
+// (v + 2^(b-1)) is exact, the scaling by 2^-b is exact, floor is exact and
+// the fma residual is an integer of at most b bits, so nothing rounds.
+__device__ __forceinline__ double balf(double v, double two_b, double inv_two_b, double& carry)
 {
-    const long long d1 = bal((long long)tr + q1, b, q1);
-    const long long d2 = bal((long long)hr - k * d1 + q2, b, q2);
-    return bal(-k * d2 + q3, b, q3);
+    const double q = floor(fma(v, inv_two_b, 0.5));
+    carry = q;
+    return fma(-q, two_b, v);
+}
+
+// Digit widths and weight indices walk a run incrementally: with
+// f_m = (-n*m) mod L, b_m = floor(n/L) + (f_m < n mod L) and
+// f_(m+1) = f_m - (n mod L) (+L when it would go negative).
+struct DigitWalk {
+    unsigned f, nr, L, bq;
+    __device__ __forceinline__ DigitWalk(const Params& p, unsigned m0)
+        : f(fidx(p, m0)), nr((unsigned)(p.n & (p.L - 1))), L(p.L), bq((unsigned)(p.n >> p.logL))
+    {
+    }
+    __device__ __forceinline__ bool wide() const { return f < nr; }  // b_m = bq + 1
+    __device__ __forceinline__ void next() { f = f >= nr ? f - nr : f + L - nr; }
+};
+
+// One digit through the three carry passes: T_lo, S = T_hi - k*T_lo,
+// R = S_hi - k*S_lo, all exact in FP64 (|T|, |k*d| < 2^52 while the FFT
+// round-off is below the guard).  Returns the digit of R; advances the
+// carries (exact integers held in doubles).
+__device__ __forceinline__ double carry3f(double tr, double hr, double kd, double two_b, double inv_two_b, double& q1,
+                                          double& q2, double& q3)
+{
+    const double d1 = balf(tr + q1, two_b, inv_two_b, q1);
+    const double d2 = balf(fma(-kd, d1, hr) + q2, two_b, inv_two_b, q2);
+    return balf(fma(-kd, d2, q3), two_b, inv_two_b, q3);
 }
 
 // Adjustment a run receives from its predecessor: c2in is the first fold's
@@ -144,12 +182,19 @@ __device__ __forceinline__ long long carry3(double tr, double hr, long long k, i
 template <int W>
 __device__ __forceinline__ void ripple(const Params& p, unsigned m0, long long c2in, long long extra, long long (&nu)[W])
 {
+    int bw[W];
+    DigitWalk walk(p, m0);
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        bw[j] = (int)walk.bq + (walk.wide() ? 1 : 0);
+        walk.next();
+    }
     long long rem = c2in;
     long long a[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) {
         long long c;
-        const long long d = bal(rem, width(p, m0 + j), c);
+        const long long d = bal(rem, bw[j], c);
         a[j] = -p.k * d;
         rem = c;
     }
@@ -158,7 +203,7 @@ __device__ __forceinline__ void ripple(const Params& p, unsigned m0, long long c
 #pragma unroll
     for (int j = 0; j < W; ++j) {
         long long cc;
-        nu[j] = bal(a[j] + c, width(p, m0 + j), cc);
+        nu[j] = bal(a[j] + c, bw[j], cc);
         c = cc;
     }
     if (rem != 0 || c != 0) atomicOr(p.status, 1);
@@ -224,50 +269,76 @@ __global__ __launch_bounds__(TPB) void k_row(Params p)
 // ------------------------------------------------------------- flat carry
 
 // Unweight, round (with max round-off), three carry passes per run of RUN
-// digits, in-block run hand-off, re-weight.  The first run of each block is
+// digits, in-block run hand-off, re-weight.  The block's TPB*RUN digits are
+// staged through LDS so global loads and stores are coalesced; in LDS run t
+// occupies slots t*(RUN+1) .. +RUN-1 (one pad slot per run keeps the
+// per-thread accesses conflict-free).  The first run of each block is
 // finished by k_fixup.
 template <int RUN, int TPB>
 __global__ __launch_bounds__(TPB) void k_carry(Params p)
 {
+    constexpr int SPAN = TPB * RUN;
+    __shared__ double2 buf[SPAN + TPB];
     __shared__ long long sq[3][TPB];
+    __shared__ double s_err[TPB / 64];
     const unsigned tid = threadIdx.x;
-    const unsigned m0 = (blockIdx.x * TPB + tid) * RUN;
+    const unsigned base = blockIdx.x * SPAN;
+    for (int i = tid; i < SPAN; i += TPB) buf[i + i / RUN] = p.z[base + i];
+    __syncthreads();
+
+    const unsigned m0 = base + tid * RUN;
+    double2* run = buf + tid * (RUN + 1);
     const long long k = p.k;
-    long long x[RUN];
-    long long q1 = 0, q2 = 0, q3 = 0;
+    const double kd = (double)p.k;
+    DigitWalk walk(p, m0);
+    const double tb0 = ldexp(1.0, (int)walk.bq), tb1 = 2.0 * tb0;
+    const double itb0 = 1.0 / tb0, itb1 = 0.5 * itb0;
+    double q1 = 0, q2 = 0, q3 = 0;
     double emax = 0.0;
 #pragma unroll
     for (int j = 0; j < RUN; ++j) {
         const unsigned m = m0 + j;
-        const double2 v = cmul(p.z[m], inv_weight(p, m));
+        const double2 v = cmul(run[j], inv_weight_f(p, m, walk.f));
         const double tr = rint(v.x), hr = rint(v.y);
         emax = fmax(emax, fmax(fabs(v.x - tr), fabs(v.y - hr)));
-        x[j] = carry3(tr, hr, k, width(p, m), q1, q2, q3);
+        const bool wd = walk.wide();
+        run[j].x = carry3f(tr, hr, kd, wd ? tb1 : tb0, wd ? itb1 : itb0, q1, q2, q3);
+        walk.next();
     }
-    sq[0][tid] = q1;
-    sq[1][tid] = q2;
-    sq[2][tid] = q3;
+    sq[0][tid] = (long long)q1;
+    sq[1][tid] = (long long)q2;
+    sq[2][tid] = (long long)q3;
     __syncthreads();
+    long long nu[RUN];
     if (tid > 0) {
-        long long nu[RUN];
         ripple<RUN>(p, m0, sq[1][tid - 1] - k * sq[0][tid - 1], sq[2][tid - 1], nu);
+    } else {
 #pragma unroll
-        for (int j = 0; j < RUN; ++j) x[j] += nu[j];
+        for (int j = 0; j < RUN; ++j) nu[j] = 0;
     }
     if (tid == TPB - 1) {
-        p.carries[3 * blockIdx.x + 0] = q1;
-        p.carries[3 * blockIdx.x + 1] = q2;
-        p.carries[3 * blockIdx.x + 2] = q3;
+        p.carries[3 * blockIdx.x + 0] = sq[0][tid];
+        p.carries[3 * blockIdx.x + 1] = sq[1][tid];
+        p.carries[3 * blockIdx.x + 2] = sq[2][tid];
     }
+    DigitWalk walk2(p, m0);
 #pragma unroll
     for (int j = 0; j < RUN; ++j) {
-        const unsigned m = m0 + j;
-        const double2 w = fwd_weight(p, m);
-        const double xd = (double)x[j];
-        p.z[m] = make_double2(xd * w.x, xd * w.y);
+        const double2 w = fwd_weight_f(p, m0 + j, walk2.f);
+        walk2.next();
+        const double xd = run[j].x + (double)nu[j];
+        run[j] = make_double2(xd * w.x, xd * w.y);
     }
     emax = wave_max(emax);
-    if ((tid & 63) == 0) atomicMax(p.err, (unsigned long long)__double_as_longlong(emax));
+    if ((tid & 63) == 0) s_err[tid / 64] = emax;
+    __syncthreads();
+    for (int i = tid; i < SPAN; i += TPB) p.z[base + i] = buf[i + i / RUN];
+    if (tid == 0) {
+        double e = s_err[0];
+#pragma unroll
+        for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
+        atomicMax(p.err, (unsigned long long)__double_as_longlong(e));
+    }
 }
 
 // Block boundaries: the first run of block w takes the carries of block
@@ -306,34 +377,26 @@ __global__ void k_fixup(Params p, unsigned nblocks)
     if (w == 0) *p.xhi = p.carries[3 * (nblocks - 1) + 2];
 }
 
-// ------------------------------------------------------------- fused outer pass
+// ------------------------------------------------------------- split outer pass
 
-__device__ __forceinline__ unsigned ld_acquire(const unsigned* f)
-{
-    return __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Block b owns columns [b*NT, b*NT+NT): for each row r its digits
+// Column block b owns columns [b*NT, b*NT+NT): for each row r its digits
 // r*C + b*NT .. +NT-1 form one run.  The predecessor of run (b, r) is
-// (b-1, r), or (nb-1, r-1) for b = 0; run (0, 0) takes the top rule.
+// (b-1, r), or (nb-1, r-1) for b = 0; run (0, 0) takes the top rule.  Run
+// data between the two kernels is block-major: run (b, r) has its NT int32
+// digits at dig + (b*R + r)*NT and its three carry-outs at runc + (b*R + r)*3.
+
+// Inverse column DFT, unweight, round (max round-off), own carries of every
+// run; writes the runs' digits (before their carry-in) and carry-outs.
 template <int R, int NT, int TPB>
-__global__ __launch_bounds__(TPB) void k_outer(Params p)
+__global__ __launch_bounds__(TPB) void k_inv_carry(Params p)
 {
     using Lay = LdsLayout<R, NT>;
-    constexpr int RPT = R / TPB;  // runs per thread
+    constexpr int RPT = R / TPB;
     static_assert(R % TPB == 0, "runs must split evenly over threads");
     __shared__ double2 lds[Lay::kSlots];
-    __shared__ unsigned s_blk;
     __shared__ double s_err[TPB / 64];
-
-    const unsigned nb = p.C / NT;
-    if (threadIdx.x == 0) s_blk = atomicAdd(p.ticket, 1u) % nb;
-    __syncthreads();
-    const unsigned b = s_blk;
+    const unsigned b = blockIdx.x;
     const unsigned c0 = b * NT;
-    const long long k = p.k;
-
-    // 1. inverse column DFT into LDS (natural row order)
     const double2* __restrict__ z = p.z;
     dft<R, NT, TPB, true, false, true>(
         lds, p.tw,
@@ -344,81 +407,34 @@ __global__ __launch_bounds__(TPB) void k_outer(Params p)
         [&](int tr, int idx, double2 v) { lds[Lay::at(tr, idx)] = v; });
     __syncthreads();
 
-    // 2. own carries of each run
-    long long x[RPT][NT];
     double emax = 0.0;
+    const double kd = (double)p.k;
+    const double tb0 = ldexp(1.0, (int)(p.n >> p.logL)), tb1 = 2.0 * tb0;
+    const double itb0 = 1.0 / tb0, itb1 = 0.5 * itb0;
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
         const unsigned r = threadIdx.x + u * TPB;
         const unsigned m0 = r * p.C + c0;
-        long long q1 = 0, q2 = 0, q3 = 0;
+        double q1 = 0, q2 = 0, q3 = 0;
+        DigitWalk walk(p, m0);
+        int dg[NT];
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
-            const unsigned m = m0 + j;
-            const double2 v = cmul(lds[Lay::at(j, r)], inv_weight(p, m));
+            const double2 v = cmul(lds[Lay::at(j, r)], inv_weight_f(p, m0 + j, walk.f));
             const double tr = rint(v.x), hr = rint(v.y);
             emax = fmax(emax, fmax(fabs(v.x - tr), fabs(v.y - hr)));
-            x[u][j] = carry3(tr, hr, k, width(p, m), q1, q2, q3);
+            const bool wd = walk.wide();
+            dg[j] = (int)carry3f(tr, hr, kd, wd ? tb1 : tb0, wd ? itb1 : itb0, q1, q2, q3);
+            walk.next();
         }
-        long long* rc = p.runc + 3 * ((size_t)b * R + r);
-        rc[0] = q1;
-        rc[1] = q2;
-        rc[2] = q3;
-    }
-    // 3. publish, then wait for the predecessor block
-    __threadfence();
-    __syncthreads();
-    const unsigned pred = b == 0 ? nb - 1 : b - 1;
-    if (threadIdx.x == 0) {
-        const unsigned mine = __hip_atomic_fetch_add(p.flags + b, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) + 1;
-        // bounded wait: every block publishes before it waits, so the bound
-        // is never reached in a healthy launch; if it is, the status word
-        // reports it and the kernel still exits
-        unsigned spins = 0;
-        while (ld_acquire(p.flags + pred) < mine) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++spins > (1u << 22)) {
-                atomicOr(p.status, 4);
-                break;
-            }
-        }
-    }
-    __syncthreads();
-    __threadfence();
-
-    // 4. fold the predecessor's carries into each run's first digits (only
-    //    block 0 reads X_hi: it is also the block that replaces it)
-    const long long xh = b == 0 ? *p.xhi : 0;
+        const size_t run = (size_t)b * R + r;
+        int4* dst = reinterpret_cast<int4*>(p.dig + run * NT);
 #pragma unroll
-    for (int u = 0; u < RPT; ++u) {
-        const unsigned r = threadIdx.x + u * TPB;
-        const unsigned m0 = r * p.C + c0;
-        long long c2in, extra;
-        if (b == 0 && r == 0) {
-            const long long* rc = p.runc + 3 * ((size_t)(nb - 1) * R + (R - 1));
-            c2in = __hip_atomic_load(rc + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            extra = __hip_atomic_load(rc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + xh * xh;
-        } else {
-            const size_t src = b == 0 ? (size_t)(nb - 1) * R + (r - 1) : (size_t)(b - 1) * R + r;
-            const long long* rc = p.runc + 3 * src;
-            const long long c1 = __hip_atomic_load(rc + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const long long c2 = __hip_atomic_load(rc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            c2in = c2 - k * c1;
-            extra = __hip_atomic_load(rc + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        long long nu[NT];
-        ripple<NT>(p, m0, c2in, extra, nu);
-#pragma unroll
-        for (int j = 0; j < NT; ++j) {
-            const unsigned m = m0 + j;
-            const double2 w = fwd_weight(p, m);
-            const double xd = (double)(x[u][j] + nu[j]);
-            lds[Lay::at(j, r)] = make_double2(xd * w.x, xd * w.y);
-        }
-    }
-    if (b == 0 && threadIdx.x == 0) {
-        const long long* rc = p.runc + 3 * ((size_t)(nb - 1) * R + (R - 1));
-        *p.xhi = __hip_atomic_load(rc + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int j = 0; j < NT / 4; ++j) dst[j] = make_int4(dg[4 * j], dg[4 * j + 1], dg[4 * j + 2], dg[4 * j + 3]);
+        long long* rc = p.runc + 3 * run;
+        rc[0] = (long long)q1;
+        rc[1] = (long long)q2;
+        rc[2] = (long long)q3;
     }
     emax = wave_max(emax);
     if ((threadIdx.x & 63) == 0) s_err[threadIdx.x / 64] = emax;
@@ -429,8 +445,61 @@ __global__ __launch_bounds__(TPB) void k_outer(Params p)
         for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
         atomicMax(p.err, (unsigned long long)__double_as_longlong(e));
     }
+}
 
-    // 5. forward column DFT from LDS, twiddle, store
+// Each run takes its predecessor's carries into its first digits, the
+// digits are weighted, and the block runs its forward column DFT.  Block 0
+// also replaces X_hi with the top run's third carry (it is the only reader
+// of the old value: X_hi^2 enters run (0, 0)).
+template <int R, int NT, int TPB>
+__global__ __launch_bounds__(TPB) void k_adjust_fwd(Params p)
+{
+    using Lay = LdsLayout<R, NT>;
+    constexpr int RPT = R / TPB;
+    __shared__ double2 lds[Lay::kSlots];
+    const unsigned nb = p.C / NT;
+    const unsigned b = blockIdx.x;
+    const unsigned c0 = b * NT;
+    const long long k = p.k;
+    const long long* top = p.runc + 3 * ((size_t)(nb - 1) * R + (R - 1));
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        const unsigned m0 = r * p.C + c0;
+        long long c2in, extra;
+        if (b == 0 && r == 0) {
+            const long long xh = *p.xhi;
+            c2in = top[0];
+            extra = top[1] + xh * xh;
+        } else {
+            const long long* rc = p.runc + 3 * (b == 0 ? (size_t)(nb - 1) * R + (r - 1) : (size_t)(b - 1) * R + r);
+            c2in = rc[1] - k * rc[0];
+            extra = rc[2];
+        }
+        long long nu[NT];
+        ripple<NT>(p, m0, c2in, extra, nu);
+        const int4* src = reinterpret_cast<const int4*>(p.dig + ((size_t)b * R + r) * NT);
+        int dg[NT];
+#pragma unroll
+        for (int j = 0; j < NT / 4; ++j) {
+            const int4 q = src[j];
+            dg[4 * j] = q.x;
+            dg[4 * j + 1] = q.y;
+            dg[4 * j + 2] = q.z;
+            dg[4 * j + 3] = q.w;
+        }
+        DigitWalk walk(p, m0);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const double2 w = fwd_weight_f(p, m0 + j, walk.f);
+            walk.next();
+            const double xd = (double)dg[j] + (double)nu[j];
+            lds[Lay::at(j, r)] = make_double2(xd * w.x, xd * w.y);
+        }
+    }
+    // run (0, 0) -- thread 0, u = 0 -- read the old X_hi above, in this thread
+    if (b == 0 && threadIdx.x == 0) *p.xhi = top[2];
+    __syncthreads();
     dft<R, NT, TPB, false, true, false>(
         lds, p.tw, [&](int tr, int idx) { return lds[Lay::at(tr, idx)]; },
         [&](int tr, int idx, double2 v) {
@@ -464,8 +533,6 @@ __global__ void k_fill_u64(unsigned long long* a, int count)
 
 // ------------------------------------------------------------- host side
 
-static constexpr int kRun = 16;
-static constexpr int kCarryTpb = 64;
 static constexpr int kOuterNT = 8;
 
 double max_bits_per_digit(uint32_t logL)
@@ -549,7 +616,7 @@ bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::strin
 }
 
 static Params make_params(const Plan& pl, double2* z, const DeviceTables* t, long long* carries, long long* runc,
-                          unsigned* flags, unsigned* ticket, long long* xhi, unsigned long long* err, int* status)
+                          int* dig, long long* xhi, unsigned long long* err, int* status)
 {
     Params p;
     p.z = z;
@@ -564,8 +631,7 @@ static Params make_params(const Plan& pl, double2* z, const DeviceTables* t, lon
     p.om_hi = t->om_hi;
     p.carries = carries;
     p.runc = runc;
-    p.flags = flags;
-    p.ticket = ticket;
+    p.dig = dig;
     p.xhi = xhi;
     p.err = err;
     p.status = status;
@@ -592,7 +658,7 @@ struct RowCfg {
     static constexpr int TPB = (NT * C / 16) < 64 ? 64 : (NT * C / 16);
 };
 
-enum class Kind { FwdCol, InvCol, Outer };
+enum class Kind { FwdCol, InvCol, InvCarry, AdjFwd };
 
 template <int R>
 static void launch_col(Kind kind, const Params& p, hipStream_t st)
@@ -602,11 +668,14 @@ static void launch_col(Kind kind, const Params& p, hipStream_t st)
     switch (kind) {
     case Kind::FwdCol: hipLaunchKernelGGL((k_fwd_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p); break;
     case Kind::InvCol: hipLaunchKernelGGL((k_inv_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p); break;
-    case Kind::Outer:
-        if constexpr (R >= 64 && R <= 1024 && NT == kOuterNT && R % TPB == 0)
-            hipLaunchKernelGGL((k_outer<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
-        else
-            throw std::runtime_error("no fused outer kernel for this column length");
+    case Kind::InvCarry:
+    case Kind::AdjFwd:
+        if constexpr (R >= 64 && R <= 1024 && NT == kOuterNT && R % TPB == 0) {
+            if (kind == Kind::InvCarry) hipLaunchKernelGGL((k_inv_carry<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
+            else hipLaunchKernelGGL((k_adjust_fwd<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
+        } else {
+            throw std::runtime_error("no split outer kernels for this column length");
+        }
         break;
     }
 }
@@ -647,28 +716,11 @@ static void dispatch_row(const Params& p, hipStream_t st)
     }
 }
 
-template <int R>
-static int outer_max_blocks_per_cu()
+template <int RUN, int TPB>
+static void launch_carry_fixup(const Params& p, unsigned blocks, bool carry, bool fixup, hipStream_t st)
 {
-    constexpr int NT = ColCfg<R>::NT, TPB = ColCfg<R>::TPB;
-    if constexpr (R >= 64 && R <= 1024 && NT == kOuterNT && R % TPB == 0) {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_outer<R, NT, TPB>, TPB, 0) != hipSuccess) return 0;
-        return nb;
-    }
-    return 0;
-}
-
-static int outer_blocks_per_cu(uint32_t R)
-{
-    switch (R) {
-    case 64: return outer_max_blocks_per_cu<64>();
-    case 128: return outer_max_blocks_per_cu<128>();
-    case 256: return outer_max_blocks_per_cu<256>();
-    case 512: return outer_max_blocks_per_cu<512>();
-    case 1024: return outer_max_blocks_per_cu<1024>();
-    default: return 0;
-    }
+    if (carry) hipLaunchKernelGGL((k_carry<RUN, TPB>), dim3(blocks), dim3(TPB), 0, st, p);
+    if (fixup) hipLaunchKernelGGL((k_fixup<RUN, TPB>), dim3((blocks + 63) / 64), dim3(64), 0, st, p, blocks);
 }
 
 // Correctly rounded tables from long double (64-bit mantissa) arithmetic.
@@ -718,26 +770,21 @@ Engine::Engine(const Plan& p, int device, bool allow_fused) : plan_(p), device_(
     std::vector<double2> tw, twl_lo, twl_hi, om_lo, om_hi;
     std::vector<double> pw_lo, pw_hi, ipw_lo, ipw_hi;
     host_tables(p, tw, twl_lo, twl_hi, om_lo, om_hi, pw_lo, pw_hi, ipw_lo, ipw_hi);
-    carry_blocks_ = p.L / (kRun * kCarryTpb);
+    run_ = ripple_digits(p) <= 8 ? 8 : 16;
+    carry_tpb_ = p.L >= (uint32_t)run_ * 256u * 16u ? 256 : 64;
+    carry_blocks_ = p.L / (run_ * carry_tpb_);
 
-    // The fused pass needs: a kernel for this column length, runs (NT digits)
-    // long enough for any carry-in to settle, and every block resident at
-    // once (blocks spin on their predecessor's flag).
-    if (allow_fused && ripple_digits(p) <= kOuterNT) {
-        int bpc = outer_blocks_per_cu(p.R);
-        int cus = 0;
-        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
-        const unsigned nb = p.C / kOuterNT;
-        if (bpc > 0 && (long long)bpc * cus >= (long long)nb) plan_.fused = true;
-    }
-    const unsigned outer_blocks = p.C / kOuterNT;
+    // The split outer pass needs kernels for this column length and runs
+    // (NT digits) long enough for any carry-in to settle.
+    plan_.fused = allow_fused && ripple_digits(p) <= kOuterNT && p.R >= 64 && p.R <= 1024;
+    const size_t runs = plan_.fused ? (size_t)p.L / kOuterNT : 0;
 
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t off = 0;
     const size_t o_z = off; off += al(sizeof(double2) * p.L);
     const size_t o_car = off; off += al(sizeof(long long) * 3 * carry_blocks_);
-    const size_t o_runc = off; off += al(plan_.fused ? sizeof(long long) * 3 * (size_t)p.R * outer_blocks : 0);
-    const size_t o_flags = off; off += al(sizeof(unsigned) * (outer_blocks + 1));
+    const size_t o_runc = off; off += al(sizeof(long long) * 3 * runs);
+    const size_t o_rdig = off; off += al(sizeof(int) * kOuterNT * runs);
     const size_t o_xhi = off; off += al(sizeof(long long));
     const size_t o_err = off; off += al(sizeof(unsigned long long) * kGraphChunk);
     const size_t o_st = off; off += al(sizeof(int));
@@ -758,8 +805,7 @@ Engine::Engine(const Plan& p, int device, bool allow_fused) : plan_(p), device_(
     z_ = reinterpret_cast<double2*>(b + o_z);
     carries_ = reinterpret_cast<long long*>(b + o_car);
     runc_ = reinterpret_cast<long long*>(b + o_runc);
-    flags_ = reinterpret_cast<unsigned*>(b + o_flags);
-    ticket_ = flags_ + outer_blocks;
+    rdig_ = reinterpret_cast<int*>(b + o_rdig);
     xhi_ = reinterpret_cast<long long*>(b + o_xhi);
     err_ = reinterpret_cast<unsigned long long*>(b + o_err);
     status_ = reinterpret_cast<int*>(b + o_st);
@@ -800,9 +846,17 @@ Engine::~Engine()
     delete tabs_;
 }
 
+void Engine::carry(const Params& p, bool do_carry, bool do_fixup)
+{
+    if (run_ == 8 && carry_tpb_ == 256) launch_carry_fixup<8, 256>(p, carry_blocks_, do_carry, do_fixup, stream_);
+    else if (run_ == 8) launch_carry_fixup<8, 64>(p, carry_blocks_, do_carry, do_fixup, stream_);
+    else if (carry_tpb_ == 256) launch_carry_fixup<16, 256>(p, carry_blocks_, do_carry, do_fixup, stream_);
+    else launch_carry_fixup<16, 64>(p, carry_blocks_, do_carry, do_fixup, stream_);
+}
+
 Params Engine::params(uint64_t slot) const
 {
-    return make_params(plan_, z_, tabs_, carries_, runc_, flags_, ticket_, xhi_, err_ + slot, status_);
+    return make_params(plan_, z_, tabs_, carries_, runc_, rdig_, xhi_, err_ + slot, status_);
 }
 
 void Engine::upload(const std::vector<int32_t>& digits, int64_t xhi)
@@ -844,21 +898,18 @@ void Engine::enqueue_chunk(int g)
     for (int i = 0; i + 1 < g; ++i) {
         const Params p = params((uint64_t)i);
         if (plan_.fused) {
-            dispatch_col(Kind::Outer, p, stream_);
+            dispatch_col(Kind::InvCarry, p, stream_);
+            dispatch_col(Kind::AdjFwd, p, stream_);
         } else {
             dispatch_col(Kind::InvCol, p, stream_);
-            hipLaunchKernelGGL((k_carry<kRun, kCarryTpb>), dim3(carry_blocks_), dim3(kCarryTpb), 0, stream_, p);
-            hipLaunchKernelGGL((k_fixup<kRun, kCarryTpb>), dim3((carry_blocks_ + 63) / 64), dim3(64), 0, stream_, p,
-                               carry_blocks_);
+            carry(p, true, true);
             dispatch_col(Kind::FwdCol, p, stream_);
         }
         dispatch_row(p, stream_);
     }
     const Params pl = params((uint64_t)(g - 1));
     dispatch_col(Kind::InvCol, pl, stream_);
-    hipLaunchKernelGGL((k_carry<kRun, kCarryTpb>), dim3(carry_blocks_), dim3(kCarryTpb), 0, stream_, pl);
-    hipLaunchKernelGGL((k_fixup<kRun, kCarryTpb>), dim3((carry_blocks_ + 63) / 64), dim3(64), 0, stream_, pl,
-                       carry_blocks_);
+    carry(pl, true, true);
 }
 
 void Engine::build_graph(int iters)
@@ -919,7 +970,7 @@ void Engine::kernel_times(int reps, double* ms)
     check(hipEventCreate(&a), "hipEventCreate");
     check(hipEventCreate(&b), "hipEventCreate");
     for (int kind = 0; kind < kNumKernels; ++kind) {
-        if (kind == 5 && !plan_.fused) {
+        if (kind >= 5 && !plan_.fused) {
             ms[kind] = 0.0;
             continue;
         }
@@ -931,14 +982,10 @@ void Engine::kernel_times(int reps, double* ms)
                 case 0: dispatch_col(Kind::FwdCol, p, stream_); break;
                 case 1: dispatch_row(p, stream_); break;
                 case 2: dispatch_col(Kind::InvCol, p, stream_); break;
-                case 3:
-                    hipLaunchKernelGGL((k_carry<kRun, kCarryTpb>), dim3(carry_blocks_), dim3(kCarryTpb), 0, stream_, p);
-                    break;
-                case 4:
-                    hipLaunchKernelGGL((k_fixup<kRun, kCarryTpb>), dim3((carry_blocks_ + 63) / 64), dim3(64), 0,
-                                       stream_, p, carry_blocks_);
-                    break;
-                default: dispatch_col(Kind::Outer, p, stream_); break;
+                case 3: carry(p, true, false); break;
+                case 4: carry(p, false, true); break;
+                case 5: dispatch_col(Kind::InvCarry, p, stream_); break;
+                default: dispatch_col(Kind::AdjFwd, p, stream_); break;
                 }
             }
             if (pass == 1) check(hipEventRecord(b, stream_), "hipEventRecord");

@@ -145,12 +145,20 @@ __device__ __forceinline__ void dftp(double2* v)
     else dft2<INV>(v);
 }
 
+#ifndef PL_MAX_RADIX
+#define PL_MAX_RADIX 16
+#endif
+
 template <int N>
 struct Radix {
-    // radix of the stage whose span is NS
+    // radix of the stage whose span is NS (largest power of two up to
+    // PL_MAX_RADIX that still fits)
     static constexpr int of(int NS)
     {
-        return (N / NS) >= 16 ? 16 : ((N / NS) >= 8 ? 8 : ((N / NS) >= 4 ? 4 : 2));
+        return (N / NS) >= 16 && PL_MAX_RADIX >= 16 ? 16
+             : (N / NS) >= 8 && PL_MAX_RADIX >= 8   ? 8
+             : (N / NS) >= 4                        ? 4
+                                                    : 2;
     }
 };
 
