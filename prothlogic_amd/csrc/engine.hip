@@ -391,8 +391,7 @@ template <int R, int NT, int TPB>
 __global__ __launch_bounds__(TPB) void k_inv_carry(Params p)
 {
     using Lay = LdsLayout<R, NT>;
-    constexpr int RPT = R / TPB;
-    static_assert(R % TPB == 0, "runs must split evenly over threads");
+    constexpr int RPT = (R + TPB - 1) / TPB;
     __shared__ double2 lds[Lay::kSlots];
     __shared__ double s_err[TPB / 64];
     const unsigned b = blockIdx.x;
@@ -414,6 +413,7 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params p)
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
         const unsigned r = threadIdx.x + u * TPB;
+        if (r >= (unsigned)R) break;
         const unsigned m0 = r * p.C + c0;
         double q1 = 0, q2 = 0, q3 = 0;
         DigitWalk walk(p, m0);
@@ -455,7 +455,7 @@ template <int R, int NT, int TPB>
 __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params p)
 {
     using Lay = LdsLayout<R, NT>;
-    constexpr int RPT = R / TPB;
+    constexpr int RPT = (R + TPB - 1) / TPB;
     __shared__ double2 lds[Lay::kSlots];
     const unsigned nb = p.C / NT;
     const unsigned b = blockIdx.x;
@@ -465,6 +465,7 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params p)
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
         const unsigned r = threadIdx.x + u * TPB;
+        if (r >= (unsigned)R) break;
         const unsigned m0 = r * p.C + c0;
         long long c2in, extra;
         if (b == 0 && r == 0) {
@@ -646,16 +647,22 @@ static Params make_params(const Plan& pl, double2* z, const DeviceTables* t, lon
     return p;
 }
 
+// One largest-radix butterfly per thread per stage, within [64, 1024].
+constexpr int tpb_for(int points)
+{
+    return points / PL_MAX_RADIX < 64 ? 64 : (points / PL_MAX_RADIX > 1024 ? 1024 : points / PL_MAX_RADIX);
+}
+
 template <int R>
 struct ColCfg {
     static constexpr int NT = R <= 1024 ? 8 : 4;
-    static constexpr int TPB = (NT * R / 16) < 64 ? 64 : (NT * R / 16);
+    static constexpr int TPB = tpb_for(NT * R);
 };
 
 template <int C>
 struct RowCfg {
     static constexpr int NT = (1024 / C) > 1 ? (1024 / C) : 1;
-    static constexpr int TPB = (NT * C / 16) < 64 ? 64 : (NT * C / 16);
+    static constexpr int TPB = tpb_for(NT * C);
 };
 
 enum class Kind { FwdCol, InvCol, InvCarry, AdjFwd };
@@ -670,7 +677,7 @@ static void launch_col(Kind kind, const Params& p, hipStream_t st)
     case Kind::InvCol: hipLaunchKernelGGL((k_inv_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p); break;
     case Kind::InvCarry:
     case Kind::AdjFwd:
-        if constexpr (R >= 64 && R <= 1024 && NT == kOuterNT && R % TPB == 0) {
+        if constexpr (R >= 64 && R <= 1024 && NT == kOuterNT) {
             if (kind == Kind::InvCarry) hipLaunchKernelGGL((k_inv_carry<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
             else hipLaunchKernelGGL((k_adjust_fwd<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
         } else {

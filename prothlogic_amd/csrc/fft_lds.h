@@ -146,7 +146,7 @@ __device__ __forceinline__ void dftp(double2* v)
 }
 
 #ifndef PL_MAX_RADIX
-#define PL_MAX_RADIX 16
+#define PL_MAX_RADIX 8
 #endif
 
 template <int N>
