@@ -69,19 +69,37 @@ def time_squarings(core, k, n, steps, warmup):
     emax = c.square(steps)
     t1 = time.perf_counter()
     r64 = c.res64()
-    kt, traffic = c.kernel_times(reps=20)
-    info = dict(fft_len=c.fft_len, rows=c.info.rows, cols=c.info.cols, bits_per_digit=round(c.info.bits_per_digit, 4))
+    kt, _ = c.kernel_times(reps=20)
+    info = dict(fft_len=c.fft_len, rows=c.info.rows, cols=c.info.cols, bits_per_digit=round(c.info.bits_per_digit, 4),
+                kernels_per_squaring=c.info.passes)
     c.close()
-    return dict(seconds=t1 - t0, max_err=emax, res64=r64, kernel_ms=kt, pass_bytes=traffic, a=a, **info)
+    nt = 0 if info["kernels_per_squaring"] == 5 else (8 if kt.get("inv_carry", 0) else 0)
+    return dict(seconds=t1 - t0, max_err=emax, res64=r64, kernel_ms=kt, a=a, nt=nt, **info)
 
 
-def roofline(kt: dict, pass_bytes: int):
-    """Dominant kernel's achieved algorithmic GB/s vs the HBM peak."""
-    full = {k: v for k, v in kt.items() if k != "fixup"}
-    dom = max(full, key=full.get)
-    achieved = pass_bytes / (full[dom] * 1e-3) / 1e9
+def kernel_bytes(L: int, nt: int) -> dict:
+    """Algorithmic HBM bytes one launch of each kernel moves for a length-L
+    residue: complex FP64 points are 16 B (read and/or written once); the
+    split outer pass hands 4 B int32 digits and 24 B of run carries per run
+    of nt digits from its first to its second kernel."""
+    run = 24.0 / nt if nt else 0.0
+    return {"fwd_col": 32 * L, "row_square": 32 * L, "inv_col": 32 * L, "carry": 32 * L,
+            "inv_carry": int((16 + 4 + run) * L), "adjust_fwd": int((4 + run + 16) * L)}
+
+
+STEADY = ("inv_carry", "adjust_fwd", "row_square")
+UNFUSED = ("inv_col", "carry", "fwd_col", "row_square")
+
+
+def roofline(kt: dict, L: int, nt: int):
+    """Dominant steady-state kernel's achieved algorithmic GB/s vs the HBM peak."""
+    names = STEADY if nt else UNFUSED
+    kb = kernel_bytes(L, nt)
+    dom = max(names, key=lambda k: kt[k])
+    achieved = kb[dom] / (kt[dom] * 1e-3) / 1e9
+    per_sq = sum(kb[k] for k in names)
     return dom, dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None)
+                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None), kb[dom], per_sq
 
 
 def measured_traffic(kernel: str, workload: str):
@@ -154,7 +172,7 @@ def main():
     r2 = time_squarings(core, k2, n2, args.steps, args.warmup)
     us2 = r2["seconds"] / args.steps * 1e6
     g2 = golden_res64(k2, n2, args.steps)
-    dom, roof = roofline(r2["kernel_ms"], r2["pass_bytes"])
+    dom, roof, dom_bytes, sq_bytes = roofline(r2["kernel_ms"], r2["fft_len"], r2["nt"])
     roof["traffic"] = measured_traffic(dom, "c2_n2p21")
     log(f"[bench] c2: {us2:.2f} us/squaring, max_err={r2['max_err']:.4g}, kernels(ms)={r2['kernel_ms']}")
 
@@ -166,16 +184,15 @@ def main():
         r3 = time_squarings(core, k3, n3, s3, 20)
         us3 = r3["seconds"] / s3 * 1e6
         g3 = golden_res64(k3, n3, s3)
-        dom3, roof3 = roofline(r3["kernel_ms"], r3["pass_bytes"])
+        dom3, roof3, dom3_bytes, sq3_bytes = roofline(r3["kernel_ms"], r3["fft_len"], r3["nt"])
         roof3["traffic"] = measured_traffic(dom3, "c3_n2p24")
-        passes = 4
-        roofline_us = 32 * passes * r3["fft_len"] / (HBM_PEAK_GBS * 1e9) * 1e6
+        roofline_us = sq3_bytes / (HBM_PEAK_GBS * 1e9) * 1e6
         extra["n2p24"] = dict(k=k3, n=n3, fft_len=r3["fft_len"], us_per_squaring=round(us3, 3), squarings=s3,
                               max_err=r3["max_err"], res64=format(r3["res64"], "016X"),
                               res64_matches_gmp=(None if g3 is None else r3["res64"] == g3),
                               kernel_ms=r3["kernel_ms"], dominant_kernel=dom3, roofline=roof3,
-                              passes_per_squaring=passes, hbm_roofline_us=round(roofline_us, 3),
-                              x_of_roofline=round(us3 / roofline_us, 2))
+                              kernels_per_squaring=r3["kernels_per_squaring"], bytes_per_squaring=sq3_bytes,
+                              hbm_roofline_us=round(roofline_us, 3), x_of_roofline=round(us3 / roofline_us, 2))
         log(f"[bench] c3: {us3:.2f} us/squaring, max_err={r3['max_err']:.4g}")
 
     cpu = None
@@ -201,12 +218,16 @@ def main():
         "data": "synthetic: seeded Proth candidate (PCG64 seed 2), residue a^k mod N built on the device",
         "config": {"workload": "c2_n2p21 single test: n=2^21, k odd in [3,2^20), 1 squaring per step",
                    "k": k2, "n": n2, "base": r2["a"], "fft_len": r2["fft_len"], "rows": r2["rows"],
-                   "cols": r2["cols"], "bits_per_digit": r2["bits_per_digit"]},
+                   "cols": r2["cols"], "bits_per_digit": r2["bits_per_digit"],
+                   "kernels_per_squaring": r2["kernels_per_squaring"]},
         "res64": format(r2["res64"], "016X"),
         "res64_matches_gmp": None if g2 is None else r2["res64"] == g2,
         "max_err": r2["max_err"],
         "kernel_ms": r2["kernel_ms"],
         "dominant_kernel": dom,
+        "dominant_kernel_bytes": dom_bytes,
+        "bytes_per_squaring": sq_bytes,
+        "hbm_roofline_us": round(sq_bytes / (HBM_PEAK_GBS * 1e9) * 1e6, 3),
         "roofline": roof,
         "cpu_baseline": cpu,
         **extra,

@@ -158,6 +158,50 @@ int pl_save_checkpoint(const char* path, uint64_t k, uint32_t n, uint64_t base, 
 int pl_load_checkpoint(const char* path, uint64_t k, uint32_t n, uint64_t* base, uint64_t* next_iter,
                        uint64_t* limbs, size_t n_limbs);
 
+/* ------------------------------------------------------------------------
+ * Continuous batching.  Replaces the reference's per-thread dispatch of one
+ * test per worker (dispatch, P/dcore.cpp:84-193, calling run_task_with_ctx,
+ * P/worker.cpp:27-70, for each k of a PROTH_FF_BATCH task :51-67) with one
+ * GPU-resident pool: candidates of uneven n are grouped by transform
+ * length, every group advances all its tests with one launch per kernel kind
+ * per squaring step (graph-captured), and finished tests free their slots
+ * for queued ones.  Tests are admitted while a slot of their length is free
+ * and a new length group fits the HBM budget.  A pool is not thread-safe.
+ * ---------------------------------------------------------------------- */
+typedef struct pl_pool pl_pool;
+
+typedef struct pl_pool_result {
+    uint64_t tag;       /* the caller's id from pl_pool_submit                 */
+    uint64_t k;
+    uint32_t n;
+    uint32_t fft_len;
+    int verdict;        /* 1 prime, 0 composite, -1 stopped early (stop_after) */
+    int gated;          /* 1: rejected by the small-prime gate, no squarings   */
+    uint64_t base;      /* a                                                   */
+    uint64_t res64;     /* low 64 bits of a^(k*2^s) mod N, s = squarings after
+                           the seed (n-1 for a full test)                      */
+    uint64_t squarings; /* device squarings incl. the seed powering            */
+    double max_err;     /* max round-off over the test                         */
+    double t_submit;    /* seconds since the pool was created                  */
+    double t_admit;
+    double t_done;
+} pl_pool_result;
+
+/* hbm_budget_bytes = 0: 64 GiB; max_slots_per_length = 0: 256;
+ * k_max = 0: 2^32-1 (smaller values allow the 8-digit carry runs). */
+int pl_pool_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots_per_length, uint64_t k_max,
+                   pl_pool** out);
+void pl_pool_destroy(pl_pool* pool);
+/* Queue one candidate.  stop_after = 0 runs the full Proth test; otherwise
+ * the test stops after that many squarings past the seed (verdict -1). */
+int pl_pool_submit(pl_pool* pool, uint64_t k, uint32_t n, uint64_t tag, uint64_t stop_after);
+/* Admit queued candidates, advance every active test by `steps` squarings
+ * (fewer for tests that finish), retire finished tests and copy up to `cap`
+ * results into out (*n_out of them; the rest stay for the next call). */
+int pl_pool_run(pl_pool* pool, uint32_t steps, pl_pool_result* out, size_t cap, size_t* n_out);
+int pl_pool_status(pl_pool* pool, size_t* queued, size_t* active, size_t* finished_unread, uint64_t* hbm_bytes);
+const char* pl_pool_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,13 @@ class PlTestResult(ctypes.Structure):
                 ("max_err", ctypes.c_double), ("fft_len", ctypes.c_uint32)]
 
 
+class PoolResult(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("k", ctypes.c_uint64), ("n", ctypes.c_uint32), ("fft_len", ctypes.c_uint32),
+                ("verdict", ctypes.c_int), ("gated", ctypes.c_int), ("base", ctypes.c_uint64),
+                ("res64", ctypes.c_uint64), ("squarings", ctypes.c_uint64), ("max_err", ctypes.c_double),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
 _LIB = None
 
 
@@ -80,6 +87,12 @@ def lib() -> ctypes.CDLL:
         "pl_proth_test": ([ctypes.POINTER(PlTestArgs), ctypes.POINTER(PlTestResult)], i32),
         "pl_save_checkpoint": ([ctypes.c_char_p, u64, u32, u64, u64, _u64p, sz], i32),
         "pl_load_checkpoint": ([ctypes.c_char_p, u64, u32, _u64p, _u64p, _u64p, sz], i32),
+        "pl_pool_create": ([i32, u64, u32, u64, ctypes.POINTER(vp)], i32),
+        "pl_pool_destroy": ([vp], None),
+        "pl_pool_submit": ([vp, u64, u32, u64, u64], i32),
+        "pl_pool_run": ([vp, u32, ctypes.POINTER(PoolResult), sz, ctypes.POINTER(sz)], i32),
+        "pl_pool_status": ([vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz), _u64p], i32),
+        "pl_pool_last_error": ([], ctypes.c_char_p),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -225,3 +238,62 @@ def load_checkpoint(path: str, k: int, n: int, nl: int):
     arr = (ctypes.c_uint64 * nl)()
     _check(lib().pl_load_checkpoint(path.encode(), k, n, ctypes.byref(b), ctypes.byref(it), arr, nl))
     return b.value, it.value, _int(arr)
+
+
+def _pcheck(rc: int):
+    if rc != PL_OK:
+        raise PlError(rc, lib().pl_pool_last_error().decode(errors="replace"))
+
+
+class Pool:
+    """Continuous batching of Proth tests on one GPU (pl_pool_*)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_slots_per_length: int = 0, k_max: int = 0):
+        h = ctypes.c_void_p()
+        _pcheck(lib().pl_pool_create(device, hbm_budget, max_slots_per_length, k_max, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pl_pool_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, k: int, n: int, tag: int = 0, stop_after: int = 0):
+        _pcheck(lib().pl_pool_submit(self.h, k, n, tag, stop_after))
+
+    def run(self, steps: int = 64, cap: int = 4096):
+        buf = (PoolResult * cap)()
+        got = ctypes.c_size_t(0)
+        _pcheck(lib().pl_pool_run(self.h, steps, buf, cap, ctypes.byref(got)))
+        return [{f: getattr(buf[i], f) for f, _ in PoolResult._fields_} for i in range(got.value)]
+
+    def status(self):
+        q, a, f = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        b = ctypes.c_uint64()
+        _pcheck(lib().pl_pool_status(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(f), ctypes.byref(b)))
+        return dict(queued=q.value, active=a.value, finished_unread=f.value, hbm_bytes=b.value)
+
+    def drain(self, steps: int = 64, on_result=None):
+        """Run until every submitted test has finished; returns all results."""
+        out = []
+        while True:
+            got = self.run(steps)
+            for r in got:
+                if on_result:
+                    on_result(r)
+            out.extend(got)
+            st = self.status()
+            if st["queued"] == 0 and st["active"] == 0 and st["finished_unread"] == 0:
+                return out

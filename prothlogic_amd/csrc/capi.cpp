@@ -2,6 +2,7 @@
 // host bignum helpers.  No exception crosses this boundary.
 #include "prothcore.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -92,7 +93,7 @@ int jacobi_small(uint64_t b, uint64_t k, uint32_t n)
 
 struct pl_ctx {
     pleng::Plan plan;
-    std::unique_ptr<pleng::Engine> eng;
+    std::unique_ptr<pleng::Group> eng;  // one slot
     std::unique_ptr<plbn::Modulus> M;
     int device = 0;
 
@@ -100,7 +101,7 @@ struct pl_ctx {
     {
         std::vector<int32_t> d;
         int64_t xhi = 0;
-        eng->download(d, xhi);
+        eng->download(0, d, xhi);
         return plbn::from_digits(*M, plan.L, d, xhi);
     }
     void set_mont_value(const plbn::Limbs& y)
@@ -108,7 +109,7 @@ struct pl_ctx {
         std::vector<int32_t> d;
         int64_t xhi = 0;
         plbn::to_digits(*M, plan.L, y, d, xhi);
-        eng->upload(d, xhi);
+        eng->upload(0, d, xhi);
     }
     plbn::Limbs value() { return plbn::from_mont(*M, mont_value()); }
     void set_value(const plbn::Limbs& x) { set_mont_value(plbn::to_mont(*M, x)); }
@@ -153,9 +154,9 @@ int pl_ctx_create(uint64_t k, uint32_t n, uint32_t fft_len, int device, pl_ctx**
         c->device = device;
         c->M = std::make_unique<plbn::Modulus>(n, k);
         const char* fz = std::getenv("PL_FUSED");
-        const bool allow_fused = !(fz && fz[0] == '0');
-        c->eng = std::make_unique<pleng::Engine>(c->plan, device, allow_fused);
-        c->plan = c->eng->plan();
+        const bool allow_split = !(fz && fz[0] == '0');
+        if (!pleng::choose_layout(c->plan, (double)n / c->plan.L, k, 1, allow_split, why)) return fail(PL_ERR_ARG, why);
+        c->eng = std::make_unique<pleng::Group>(c->plan, 1, device);
         *out = c.release();
         return PL_OK;
     });
@@ -173,7 +174,7 @@ int pl_ctx_info(const pl_ctx* ctx, pl_info* out)
     out->cols = ctx->plan.C;
     out->bits_per_digit = (double)ctx->plan.n / (double)ctx->plan.L;
     out->device_bytes = ctx->eng->device_bytes();
-    out->passes = ctx->plan.fused ? 3 : 5;
+    out->passes = ctx->plan.nt ? 3 : 5;
     return PL_OK;
 }
 
@@ -209,6 +210,7 @@ int pl_square(pl_ctx* ctx, uint64_t iters, double err_guard, double* max_err, do
         double e = ctx->eng->square(iters, err_trace);
         if (max_err) *max_err = e;
         int st = ctx->eng->status();
+        if (st) ctx->eng->clear_status();
         if (st) return fail(PL_ERR_STATE, "device carry did not settle (status " + std::to_string(st) + ")");
         if (e > err_guard)
             return fail(PL_ERR_ROUNDOFF, "round-off " + std::to_string(e) + " above guard " + std::to_string(err_guard));
@@ -441,17 +443,22 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
             }
         }
         if (!resumed) {
-            // seed a^k mod N, left to right, Montgomery form on the device
+            // seed a^k mod N on the device: x = a, then one scheduled squaring
+            // per lower bit of k, multiplied by a where the bit is set
             if (a == 0) {
                 ctx->set_value(plbn::Limbs(1, 0));
             } else {
                 ctx->set_value(plbn::Limbs(1, a));
-                int top = 63;
-                while (top > 0 && !((k >> top) & 1)) --top;
-                for (int bit = top - 1; bit >= 0; --bit) {
-                    double e = ctx->eng->square(1, nullptr);
-                    if (e > out->max_err) out->max_err = e;
-                    if ((k >> bit) & 1) ctx->set_mont_value(plbn::mul_small_mod(*ctx->M, ctx->mont_value(), a));
+                unsigned top = 0;
+                while (top < 63 && (k >> (top + 1))) ++top;
+                if (top > 0) {
+                    ctx->eng->set_slot(0, n, k, (uint32_t)a, top);
+                    ctx->eng->set_state(0, 0, top > 1);
+                    ctx->eng->enter(0);
+                    double e = top > 1 ? ctx->eng->step(top - 1, nullptr) : 0.0;
+                    e = std::max(e, ctx->eng->leave(0));
+                    out->max_err = std::max(out->max_err, e);
+                    if (ctx->eng->status()) return fail(PL_ERR_STATE, "device carry did not settle (seed)");
                 }
             }
         }

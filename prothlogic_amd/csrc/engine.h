@@ -1,5 +1,10 @@
-// engine.h -- device-resident modular squaring x <- x^2 * 2^(-2n) mod N,
-// N = k*2^n + 1, on one gfx950 GPU.  See engine.hip for the algorithm.
+// engine.h -- device-resident modular squaring of Proth residues on one
+// gfx950 GPU.  A Group holds `capacity` slots that share one transform
+// length L (and its tables) but each carry their own candidate
+// N = k*2^n + 1, residue, top scalar and schedule.  Every kernel launch
+// covers all slots of a group (blockIdx.y = slot), so a batch of candidates
+// of one length advances with one launch per kernel kind.  See engine.hip
+// for the algorithm.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -10,66 +15,106 @@
 
 namespace pleng {
 
-// Complex length L = R*C digits; R = strided (column) DFT length, C =
-// contiguous (row) DFT length.
+// Transform shape for one length: L = R*C complex points (= digits of a
+// residue), R = strided (column) DFT length, C = contiguous (row) length.
 struct Plan {
-    uint32_t n = 0;
+    uint32_t n = 0;       // candidate the plan was chosen for (single use)
     uint64_t k = 0;
     uint32_t L = 0, logL = 0, R = 0, C = 0;
-    uint32_t s = 0;      // log2 of the low half of the two-level tables
-    bool fused = false;  // steady state uses the split outer pass (3 kernels per squaring)
+    uint32_t s = 0;       // log2 of the low half of the two-level tables
+    uint32_t nt = 0;      // split outer pass with runs of nt digits (8 or 16), 0 = unfused
+    uint32_t run = 16;    // digits per thread in the flat carry kernel (8 or 16)
 };
 
-// Rules for L: smallest power of two in [2^10, 2^22] whose bits per digit
-// n/L stays within max_bits_per_digit(L) and above 8.  Returns false when
-// no length qualifies.
+// Smallest power of two L in [2^10, 2^22] with n/L <= max_bits_per_digit(L)
+// and n/L >= 8.  force_L, when nonzero, is checked and used instead.
 bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::string& why);
 double max_bits_per_digit(uint32_t logL);
+// Fills nt and run for slots whose bits per digit are at least bmin, whose
+// k is below kmax and whose per-squaring multiplier is at most mult_max.
+// Returns false when no carry layout can absorb those carries.
+bool choose_layout(Plan& p, double bmin, uint64_t kmax, uint32_t mult_max, bool allow_split, std::string& why);
 
 struct DeviceTables;
 struct Params;
 
-class Engine {
+// Per-slot constants on the device.
+struct SlotDesc {
+    unsigned long long n;
+    long long k;
+    unsigned a;       // base of the seed powering (0: no schedule multipliers)
+    unsigned topbit;  // index of the top set bit of k
+    unsigned long long total;  // squarings of the whole test (topbit + n - 1)
+};
+
+// Per-slot progress on the device (scheduled stepping only).
+struct SlotState {
+    unsigned long long carried;  // squarings completed (carried) so far
+    int active;                  // 1 while steps still apply to the slot
+    int pad;
+};
+
+class Group {
   public:
-    Engine(const Plan& p, int device, bool allow_fused = true);
-    ~Engine();
-    Engine(const Engine&) = delete;
-    Engine& operator=(const Engine&) = delete;
+    Group(const Plan& p, uint32_t capacity, int device);
+    ~Group();
+    Group(const Group&) = delete;
+    Group& operator=(const Group&) = delete;
 
     const Plan& plan() const { return plan_; }
+    uint32_t capacity() const { return cap_; }
     hipStream_t stream() const { return stream_; }
     size_t device_bytes() const { return bytes_; }
+    static size_t slot_bytes(const Plan& p);  // HBM per slot (residue + carry staging)
 
-    // Digits (variable base, balanced) + top scalar in/out of the device.
-    void upload(const std::vector<int32_t>& digits, int64_t xhi);
-    void download(std::vector<int32_t>& digits, int64_t& xhi);
+    // Slot set-up.  a = 0 disables the schedule multipliers; total = the
+    // squarings of the slot's test (used by scheduled stepping).
+    void set_slot(uint32_t slot, uint32_t n, uint64_t k, uint32_t a, uint64_t total);
+    void set_state(uint32_t slot, uint64_t carried, bool active);
 
-    // iters squarings on the device; per-iteration max round-off into
-    // err_trace (iters entries) when non-null.  Returns the max round-off.
+    // Digits (variable base, balanced) + top scalar in/out of a slot.
+    void upload(uint32_t slot, const std::vector<int32_t>& digits, int64_t xhi);
+    void download(uint32_t slot, std::vector<int32_t>& digits, int64_t& xhi);
+
+    // Plain squaring: every slot squared `iters` times (no multipliers),
+    // from and back to the digit state.  err_trace (iters entries, nullable)
+    // gets slot 0's per-iteration max round-off.  Returns the max over all.
     double square(uint64_t iters, double* err_trace);
 
-    // device status word (non-zero: a carry did not settle within a run)
-    int status();
+    // Scheduled stepping.  enter: digit state -> first squaring's FFT done;
+    // step: carry squaring i (x <- x^2 * a^bit) and start squaring i+1 for
+    // every active slot, `steps` times; leave: carry the slot's last
+    // squaring and return to the digit state.  A slot goes inactive after
+    // total-1 steps.  step() returns the max round-off of the chunk and
+    // fills per-slot maxima (capacity entries, nullable).
+    void enter(uint32_t slot);
+    double step(uint32_t steps, double* slot_err);
+    double leave(uint32_t slot);
+    std::vector<SlotState> states();
 
-    // Average duration (ms) of each kernel, timed with HIP events on this
-    // engine's stream over `reps` launches each: [0] forward column pass,
-    // [1] row pass (DFT, square, inverse DFT), [2] inverse column pass,
-    // [3] flat carry, [4] block fix-up, [5] inverse column DFT + carry
-    // (split outer pass), [6] carry-in + forward column DFT (split outer
-    // pass); [5] and [6] are 0 when the plan does not use them.  The residue
-    // is saved before and restored after.
+    int status();
+    void clear_status();
+
+    // Average duration (ms) of each kernel over all slots, timed with HIP
+    // events on this group's stream over `reps` launches: [0] forward
+    // column pass, [1] row pass, [2] inverse column pass, [3] flat carry,
+    // [4] block fix-up, [5] inverse column DFT + carry, [6] carry-in +
+    // forward column DFT ([5], [6] are 0 without the split outer pass).
+    // Slot 0's residue is saved and restored.
     void kernel_times(int reps, double* ms);
     static constexpr int kNumKernels = 7;
-
-    static constexpr int kGraphChunk = 64;
+    static constexpr int kChunk = 64;
 
   private:
-    Params params(uint64_t slot) const;
-    void carry(const Params& p, bool do_carry, bool do_fixup);
-    void enqueue_chunk(int g);
-    void build_graph(int iters);
+    Params params(uint32_t slot0, bool check_active, uint32_t err_col) const;
+    void launch(int kind, const Params& p, uint32_t slots);
+    void enqueue_plain_chunk(int g);
+    void enqueue_step_chunk(int g);
+    void build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g);
+    double read_err(int g, double* trace, double* slot_err);
 
     Plan plan_;
+    uint32_t cap_;
     int device_;
     hipStream_t stream_ = nullptr;
     size_t bytes_ = 0;
@@ -79,16 +124,16 @@ class Engine {
     long long* runc_ = nullptr;
     int* rdig_ = nullptr;
     long long* xhi_ = nullptr;
-    unsigned long long* err_ = nullptr;  // kGraphChunk slots (bit patterns of non-negative doubles)
+    unsigned long long* err_ = nullptr;  // cap x kChunk (bit patterns of non-negative doubles)
     int* status_ = nullptr;
     int32_t* digits_dev_ = nullptr;
+    SlotDesc* desc_ = nullptr;
+    SlotState* state_ = nullptr;
     DeviceTables* tabs_ = nullptr;
-    uint32_t carry_blocks_ = 0;
-    int run_ = 16;         // digits per thread in the flat carry kernel
-    int carry_tpb_ = 64;   // threads per flat carry block
-    hipGraphExec_t graph_exec_ = nullptr;
-    hipGraph_t graph_ = nullptr;
-    int graph_iters_ = 0;
+    uint32_t carry_blocks_ = 0, carry_tpb_ = 64;
+    std::vector<SlotDesc> desc_host_;
+    hipGraphExec_t plain_exec_ = nullptr, step_exec_ = nullptr;
+    hipGraph_t plain_graph_ = nullptr, step_graph_ = nullptr;
     std::vector<unsigned long long> err_host_;
 };
 

@@ -35,8 +35,15 @@
 //                  column DFT
 //   k_row        : forward row DFT -> Z*(Z + 2i*X_hi) -> inverse row DFT
 // k_fwd_col / k_inv_col / k_carry / k_fixup are the unfused pieces used to
-// enter and leave the steady state (and for lengths the fused path does not
+// enter and leave the steady state (and for lengths the split pass does not
 // cover).
+//
+// Slots.  Every launch covers the slots of a Group (blockIdx.y = slot); a
+// slot binds its own n, k, residue, carry staging, X_hi and round-off slot.
+// Scheduled stepping multiplies the second fold by a per-slot small factor
+// (x <- x^2 * a when the current bit of k is set), which runs the seed
+// exponentiation a^k on the device with the same kernels, and retires a
+// slot after its last squaring (inactive slots' blocks exit at once).
 #include "engine.h"
 #include "fft_lds.h"
 
@@ -70,8 +77,18 @@ struct Params {
     long long* xhi;
     unsigned long long* err;
     int* status;
+    const SlotDesc* desc;
+    SlotState* state;
+    size_t carr_stride;     // per-slot strides of carries / runc / err
+    size_t runc_stride;
+    unsigned err_stride;
+    unsigned slot0;         // first slot of this launch
+    int sched;              // apply per-slot multipliers and the active flag
+    // bound per slot by bind()
     unsigned long long n;
     long long k;
+    long long mult;
+    int active;
     unsigned L, logL, R, C, s, smask;
 };
 
@@ -80,6 +97,31 @@ struct DeviceTables {
     double2 *twl_lo, *twl_hi, *om_lo, *om_hi;
     double *pw_lo, *pw_hi, *ipw_lo, *ipw_hi;
 };
+
+// The slot's view of the launch parameters.
+__device__ __forceinline__ Params bind(const Params& base, unsigned y)
+{
+    Params p = base;
+    const unsigned slot = base.slot0 + y;
+    p.z += (size_t)slot * base.L;
+    p.dig += (size_t)slot * base.L;
+    p.runc += (size_t)slot * base.runc_stride;
+    p.carries += (size_t)slot * base.carr_stride;
+    p.xhi += slot;
+    p.err += (size_t)slot * base.err_stride;
+    const SlotDesc d = base.desc[slot];
+    p.n = d.n;
+    p.k = d.k;
+    p.mult = 1;
+    p.active = 1;
+    if (base.sched) {
+        const SlotState st = base.state[slot];
+        p.active = st.active;
+        const unsigned long long i = st.carried + 1;  // squaring being carried
+        if (d.a && i <= d.topbit && ((d.k >> (d.topbit - i)) & 1)) p.mult = d.a;
+    }
+    return p;
+}
 
 // ------------------------------------------------------------- helpers
 
@@ -162,25 +204,33 @@ struct DigitWalk {
 };
 
 // One digit through the three carry passes: T_lo, S = T_hi - k*T_lo,
-// R = S_hi - k*S_lo, all exact in FP64 (|T|, |k*d| < 2^52 while the FFT
-// round-off is below the guard).  Returns the digit of R; advances the
-// carries (exact integers held in doubles).
-__device__ __forceinline__ double carry3f(double tr, double hr, double kd, double two_b, double inv_two_b, double& q1,
-                                          double& q2, double& q3)
+// R = mult*(S_hi - k*S_lo), all exact in FP64 (|T|, |k*d|, |k*mult*d| stay
+// below 2^52 while the FFT round-off is below the guard).  Returns the digit
+// of R; advances the carries (exact integers held in doubles).
+__device__ __forceinline__ double carry3f(double tr, double hr, double kd, double kmd, double two_b, double inv_two_b,
+                                          double& q1, double& q2, double& q3)
 {
     const double d1 = balf(tr + q1, two_b, inv_two_b, q1);
     const double d2 = balf(fma(-kd, d1, hr) + q2, two_b, inv_two_b, q2);
-    return balf(fma(-kd, d2, q3), two_b, inv_two_b, q3);
+    return balf(fma(-kmd, d2, q3), two_b, inv_two_b, q3);
 }
 
 // Adjustment a run receives from its predecessor: c2in is the first fold's
 // carry arriving at the run's first bit (it already includes -k times the
 // predecessor's T_lo carry); it is spread over balanced digits delta_j, which
-// enter the second fold as -k*delta_j; extra is the second fold's carry-in at
-// the same bit.  The sum is normalised into W balanced digits nu_j added to
-// the run's digits.  Sets status when W digits do not absorb it.
+// enter the second fold as -k*mult*delta_j; extra is the second fold's
+// carry-in at the same bit.  The sum is normalised into W balanced digits
+// nu_j added to the run's digits.  128-bit intermediates: c2in and extra can
+// exceed 2^63 for k near 2^32.  Sets status when W digits do not absorb it.
+__device__ __forceinline__ __int128 bal128(__int128 v, int b, __int128& carry)
+{
+    const __int128 q = (v + ((__int128)1 << (b - 1))) >> b;
+    carry = q;
+    return v - q * ((__int128)1 << b);
+}
+
 template <int W>
-__device__ __forceinline__ void ripple(const Params& p, unsigned m0, long long c2in, long long extra, long long (&nu)[W])
+__device__ __forceinline__ void ripple(const Params& p, unsigned m0, __int128 c2in, __int128 extra, long long (&nu)[W])
 {
     int bw[W];
     DigitWalk walk(p, m0);
@@ -189,24 +239,33 @@ __device__ __forceinline__ void ripple(const Params& p, unsigned m0, long long c
         bw[j] = (int)walk.bq + (walk.wide() ? 1 : 0);
         walk.next();
     }
-    long long rem = c2in;
+    const long long km = p.k * p.mult;
+    __int128 rem = c2in;
     long long a[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-        long long c;
-        const long long d = bal(rem, bw[j], c);
-        a[j] = -p.k * d;
+        __int128 c;
+        const long long d = (long long)bal128(rem, bw[j], c);
+        a[j] = -km * d;
         rem = c;
     }
-    a[0] += extra;
-    long long c = 0;
+    __int128 c = extra;
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-        long long cc;
-        nu[j] = bal(a[j] + c, bw[j], cc);
+        __int128 cc;
+        nu[j] = (long long)bal128((__int128)a[j] + c, bw[j], cc);
         c = cc;
     }
     if (rem != 0 || c != 0) atomicOr(p.status, 1);
+}
+
+// Second-fold carry-in of run (0, 0): mult*(c2_top + X_hi^2), and the first
+// fold's carry-in c1_top (T_lo's carry at bit n is part of T_hi).
+__device__ __forceinline__ void top_rule(const Params& p, const long long* top, __int128& c2in, __int128& extra)
+{
+    const long long xh = *p.xhi;
+    c2in = top[0];
+    extra = ((__int128)top[1] + (__int128)xh * xh) * p.mult;
 }
 
 __device__ __forceinline__ double wave_max(double v)
@@ -219,8 +278,10 @@ __device__ __forceinline__ double wave_max(double v)
 // ------------------------------------------------------------- FFT kernels
 
 template <int R, int NT, int TPB>
-__global__ __launch_bounds__(TPB) void k_fwd_col(Params p)
+__global__ __launch_bounds__(TPB) void k_fwd_col(Params pb)
 {
+    const Params p = bind(pb, blockIdx.y);
+    if (!p.active) return;
     __shared__ double2 lds[LdsLayout<R, NT>::kSlots];
     const unsigned c0 = blockIdx.x * NT;
     const double2* __restrict__ z = p.z;
@@ -233,8 +294,10 @@ __global__ __launch_bounds__(TPB) void k_fwd_col(Params p)
 }
 
 template <int R, int NT, int TPB>
-__global__ __launch_bounds__(TPB) void k_inv_col(Params p)
+__global__ __launch_bounds__(TPB) void k_inv_col(Params pb)
 {
+    const Params p = bind(pb, blockIdx.y);
+    if (!p.active) return;
     __shared__ double2 lds[LdsLayout<R, NT>::kSlots];
     const unsigned c0 = blockIdx.x * NT;
     const double2* __restrict__ z = p.z;
@@ -250,8 +313,10 @@ __global__ __launch_bounds__(TPB) void k_inv_col(Params p)
 // Row pass: NT rows; forward C-point DFT, Z <- Z*(Z + 2i*X_hi) folded into
 // the last forward stage, inverse C-point DFT.
 template <int C, int NT, int TPB>
-__global__ __launch_bounds__(TPB) void k_row(Params p)
+__global__ __launch_bounds__(TPB) void k_row(Params pb)
 {
+    const Params p = bind(pb, blockIdx.y);
+    if (!p.active) return;
     using Lay = LdsLayout<C, NT>;
     __shared__ double2 lds[Lay::kSlots];
     const size_t base = (size_t)blockIdx.x * NT * C;
@@ -275,8 +340,10 @@ __global__ __launch_bounds__(TPB) void k_row(Params p)
 // per-thread accesses conflict-free).  The first run of each block is
 // finished by k_fixup.
 template <int RUN, int TPB>
-__global__ __launch_bounds__(TPB) void k_carry(Params p)
+__global__ __launch_bounds__(TPB) void k_carry(Params pb)
 {
+    const Params p = bind(pb, blockIdx.y);
+    if (!p.active) return;
     constexpr int SPAN = TPB * RUN;
     __shared__ double2 buf[SPAN + TPB];
     __shared__ long long sq[3][TPB];
@@ -289,7 +356,7 @@ __global__ __launch_bounds__(TPB) void k_carry(Params p)
     const unsigned m0 = base + tid * RUN;
     double2* run = buf + tid * (RUN + 1);
     const long long k = p.k;
-    const double kd = (double)p.k;
+    const double kd = (double)p.k, kmd = (double)(p.k * p.mult);
     DigitWalk walk(p, m0);
     const double tb0 = ldexp(1.0, (int)walk.bq), tb1 = 2.0 * tb0;
     const double itb0 = 1.0 / tb0, itb1 = 0.5 * itb0;
@@ -302,7 +369,7 @@ __global__ __launch_bounds__(TPB) void k_carry(Params p)
         const double tr = rint(v.x), hr = rint(v.y);
         emax = fmax(emax, fmax(fabs(v.x - tr), fabs(v.y - hr)));
         const bool wd = walk.wide();
-        run[j].x = carry3f(tr, hr, kd, wd ? tb1 : tb0, wd ? itb1 : itb0, q1, q2, q3);
+        run[j].x = carry3f(tr, hr, kd, kmd, wd ? tb1 : tb0, wd ? itb1 : itb0, q1, q2, q3);
         walk.next();
     }
     sq[0][tid] = (long long)q1;
@@ -311,7 +378,7 @@ __global__ __launch_bounds__(TPB) void k_carry(Params p)
     __syncthreads();
     long long nu[RUN];
     if (tid > 0) {
-        ripple<RUN>(p, m0, sq[1][tid - 1] - k * sq[0][tid - 1], sq[2][tid - 1], nu);
+        ripple<RUN>(p, m0, (__int128)sq[1][tid - 1] - (__int128)k * sq[0][tid - 1], sq[2][tid - 1], nu);
     } else {
 #pragma unroll
         for (int j = 0; j < RUN; ++j) nu[j] = 0;
@@ -346,22 +413,21 @@ __global__ __launch_bounds__(TPB) void k_carry(Params p)
 // part of T_hi) and c2_top + X_hi^2 (the second fold's top scalar); the new
 // X_hi is the top run's third carry.
 template <int RUN, int CTPB>
-__global__ void k_fixup(Params p, unsigned nblocks)
+__global__ void k_fixup(Params pb, unsigned nblocks)
 {
+    const Params p = bind(pb, blockIdx.y);
+    if (!p.active) return;
     const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nblocks) return;
     const unsigned m0 = w * CTPB * RUN;
     const long long k = p.k;
-    long long c2in, extra;
+    __int128 c2in, extra;
     if (w > 0) {
         const long long* c = p.carries + 3 * (w - 1);
-        c2in = c[1] - k * c[0];
+        c2in = (__int128)c[1] - (__int128)k * c[0];
         extra = c[2];
     } else {
-        const long long* c = p.carries + 3 * (nblocks - 1);
-        const long long xh = *p.xhi;
-        c2in = c[0];
-        extra = c[1] + xh * xh;
+        top_rule(p, p.carries + 3 * (nblocks - 1), c2in, extra);
     }
     long long nu[RUN];
     ripple<RUN>(p, m0, c2in, extra, nu);
@@ -388,8 +454,10 @@ __global__ void k_fixup(Params p, unsigned nblocks)
 // Inverse column DFT, unweight, round (max round-off), own carries of every
 // run; writes the runs' digits (before their carry-in) and carry-outs.
 template <int R, int NT, int TPB>
-__global__ __launch_bounds__(TPB) void k_inv_carry(Params p)
+__global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
 {
+    const Params p = bind(pb, blockIdx.y);
+    if (!p.active) return;
     using Lay = LdsLayout<R, NT>;
     constexpr int RPT = (R + TPB - 1) / TPB;
     __shared__ double2 lds[Lay::kSlots];
@@ -407,7 +475,7 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params p)
     __syncthreads();
 
     double emax = 0.0;
-    const double kd = (double)p.k;
+    const double kd = (double)p.k, kmd = (double)(p.k * p.mult);
     const double tb0 = ldexp(1.0, (int)(p.n >> p.logL)), tb1 = 2.0 * tb0;
     const double itb0 = 1.0 / tb0, itb1 = 0.5 * itb0;
 #pragma unroll
@@ -424,7 +492,7 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params p)
             const double tr = rint(v.x), hr = rint(v.y);
             emax = fmax(emax, fmax(fabs(v.x - tr), fabs(v.y - hr)));
             const bool wd = walk.wide();
-            dg[j] = (int)carry3f(tr, hr, kd, wd ? tb1 : tb0, wd ? itb1 : itb0, q1, q2, q3);
+            dg[j] = (int)carry3f(tr, hr, kd, kmd, wd ? tb1 : tb0, wd ? itb1 : itb0, q1, q2, q3);
             walk.next();
         }
         const size_t run = (size_t)b * R + r;
@@ -452,8 +520,10 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params p)
 // also replaces X_hi with the top run's third carry (it is the only reader
 // of the old value: X_hi^2 enters run (0, 0)).
 template <int R, int NT, int TPB>
-__global__ __launch_bounds__(TPB) void k_adjust_fwd(Params p)
+__global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
 {
+    const Params p = bind(pb, blockIdx.y);
+    if (!p.active) return;
     using Lay = LdsLayout<R, NT>;
     constexpr int RPT = (R + TPB - 1) / TPB;
     __shared__ double2 lds[Lay::kSlots];
@@ -467,14 +537,12 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params p)
         const unsigned r = threadIdx.x + u * TPB;
         if (r >= (unsigned)R) break;
         const unsigned m0 = r * p.C + c0;
-        long long c2in, extra;
+        __int128 c2in, extra;
         if (b == 0 && r == 0) {
-            const long long xh = *p.xhi;
-            c2in = top[0];
-            extra = top[1] + xh * xh;
+            top_rule(p, top, c2in, extra);
         } else {
             const long long* rc = p.runc + 3 * (b == 0 ? (size_t)(nb - 1) * R + (r - 1) : (size_t)(b - 1) * R + r);
-            c2in = rc[1] - k * rc[0];
+            c2in = (__int128)rc[1] - (__int128)k * rc[0];
             extra = rc[2];
         }
         long long nu[NT];
@@ -509,8 +577,9 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params p)
         });
 }
 
-__global__ void k_set_digits(Params p, const int* __restrict__ d)
+__global__ void k_set_digits(Params pb, const int* __restrict__ d)
 {
+    const Params p = bind(pb, blockIdx.y);
     const unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= p.L) return;
     const double2 w = fwd_weight(p, m);
@@ -518,8 +587,9 @@ __global__ void k_set_digits(Params p, const int* __restrict__ d)
     p.z[m] = make_double2(x * w.x, x * w.y);
 }
 
-__global__ void k_get_digits(Params p, int* __restrict__ d)
+__global__ void k_get_digits(Params pb, int* __restrict__ d)
 {
+    const Params p = bind(pb, blockIdx.y);
     const unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= p.L) return;
     const double2 v = cmul(p.z[m], inv_weight(p, m));
@@ -532,9 +602,24 @@ __global__ void k_fill_u64(unsigned long long* a, int count)
     if (i < count) a[i] = 0ull;
 }
 
+// End of a scheduled step: a slot that has carried total-1 squarings stops
+// (its last squaring is carried by the leave sequence).
+__global__ void k_advance(SlotState* st, const SlotDesc* desc, unsigned slots)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots || !st[i].active) return;
+    const unsigned long long c = st[i].carried + 1;
+    st[i].carried = c;
+    if (c + 1 >= desc[i].total) st[i].active = 0;
+}
+
 // ------------------------------------------------------------- host side
 
-static constexpr int kOuterNT = 8;
+// One largest-radix butterfly per thread per stage, within [64, 1024].
+constexpr int tpb_for(int points)
+{
+    return points / PL_MAX_RADIX < 64 ? 64 : (points / PL_MAX_RADIX > 1024 ? 1024 : points / PL_MAX_RADIX);
+}
 
 double max_bits_per_digit(uint32_t logL)
 {
@@ -554,16 +639,17 @@ static unsigned bitlen64(unsigned long long x)
     return b;
 }
 
-// Digits a carry-in can reach (see ripple): bounds the carries from the
-// worst-case coefficient magnitude L*2^(2b+1).
-static int ripple_digits(const Plan& pl)
+// Digits a run's carry-in can reach (see ripple), bounded from the
+// worst-case coefficient magnitude L*2^(2b+1): the first fold's carry-in is
+// below 2^(b + 4 + log2 L + log2 k + 1), and the normalised second-fold
+// adjustment is k*mult times that.
+static int ripple_digits(uint32_t logL, double bmin, uint64_t kmax, uint32_t mult_max)
 {
-    const unsigned bmin = pl.n >> pl.logL;
-    const unsigned bmax = bmin + 1;
-    const unsigned kb = bitlen64(pl.k);
-    const unsigned c2bits = bmax + 4 + pl.logL + kb + 1;
-    const unsigned a3bits = c2bits + kb + 1;
-    return (int)((a3bits + 1 + bmin - 1) / bmin) + 1;
+    const unsigned bq = (unsigned)bmin;
+    const unsigned kb = bitlen64(kmax);
+    const unsigned c2bits = bq + 1 + 4 + logL + kb + 1;
+    const unsigned a3bits = c2bits + kb + bitlen64(mult_max) + 1;
+    return (int)((a3bits + 1 + bq - 1) / bq) + 1;
 }
 
 bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::string& why)
@@ -612,50 +698,32 @@ bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::strin
     out.R = 1u << logR;
     out.C = L / out.R;
     out.s = (logL + 1) / 2;
-    out.fused = false;
+    out.nt = 0;
+    out.run = 16;
     return true;
 }
 
-static Params make_params(const Plan& pl, double2* z, const DeviceTables* t, long long* carries, long long* runc,
-                          int* dig, long long* xhi, unsigned long long* err, int* status)
+bool choose_layout(Plan& p, double bmin, uint64_t kmax, uint32_t mult_max, bool allow_split, std::string& why)
 {
-    Params p;
-    p.z = z;
-    p.tw = t->tw;
-    p.twl_lo = t->twl_lo;
-    p.twl_hi = t->twl_hi;
-    p.pw_lo = t->pw_lo;
-    p.pw_hi = t->pw_hi;
-    p.ipw_lo = t->ipw_lo;
-    p.ipw_hi = t->ipw_hi;
-    p.om_lo = t->om_lo;
-    p.om_hi = t->om_hi;
-    p.carries = carries;
-    p.runc = runc;
-    p.dig = dig;
-    p.xhi = xhi;
-    p.err = err;
-    p.status = status;
-    p.n = pl.n;
-    p.k = (long long)pl.k;
-    p.L = pl.L;
-    p.logL = pl.logL;
-    p.R = pl.R;
-    p.C = pl.C;
-    p.s = pl.s;
-    p.smask = (1u << pl.s) - 1;
-    return p;
-}
-
-// One largest-radix butterfly per thread per stage, within [64, 1024].
-constexpr int tpb_for(int points)
-{
-    return points / PL_MAX_RADIX < 64 ? 64 : (points / PL_MAX_RADIX > 1024 ? 1024 : points / PL_MAX_RADIX);
+    const int w = ripple_digits(p.logL, bmin, kmax, mult_max);
+    if (w > 16) {
+        why = "carries of this candidate would not settle within 16 digits";
+        return false;
+    }
+    p.run = w <= 8 ? 8 : 16;
+    p.nt = 0;
+    if (allow_split && p.R >= 64 && p.R <= 512) p.nt = w <= 8 ? 8 : 16;
+    return true;
 }
 
 template <int R>
 struct ColCfg {
     static constexpr int NT = R <= 1024 ? 8 : 4;
+    static constexpr int TPB = tpb_for(NT * R);
+};
+
+template <int R, int NT>
+struct SplitCfg {
     static constexpr int TPB = tpb_for(NT * R);
 };
 
@@ -665,69 +733,79 @@ struct RowCfg {
     static constexpr int TPB = tpb_for(NT * C);
 };
 
-enum class Kind { FwdCol, InvCol, InvCarry, AdjFwd };
+enum Kind { kFwdCol = 0, kRow = 1, kInvCol = 2, kCarry = 3, kFixup = 4, kInvCarry = 5, kAdjFwd = 6, kAdvance = 7 };
 
 template <int R>
-static void launch_col(Kind kind, const Params& p, hipStream_t st)
+static void launch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, hipStream_t st)
 {
-    constexpr int NT = ColCfg<R>::NT, TPB = ColCfg<R>::TPB;
-    dim3 grid(p.C / NT);
-    switch (kind) {
-    case Kind::FwdCol: hipLaunchKernelGGL((k_fwd_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p); break;
-    case Kind::InvCol: hipLaunchKernelGGL((k_inv_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p); break;
-    case Kind::InvCarry:
-    case Kind::AdjFwd:
-        if constexpr (R >= 64 && R <= 1024 && NT == kOuterNT) {
-            if (kind == Kind::InvCarry) hipLaunchKernelGGL((k_inv_carry<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
-            else hipLaunchKernelGGL((k_adjust_fwd<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
-        } else {
-            throw std::runtime_error("no split outer kernels for this column length");
-        }
-        break;
+    if (kind == kFwdCol || kind == kInvCol) {
+        constexpr int NT = ColCfg<R>::NT, TPB = ColCfg<R>::TPB;
+        dim3 grid(p.C / NT, slots);
+        if (kind == kFwdCol) hipLaunchKernelGGL((k_fwd_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
+        else hipLaunchKernelGGL((k_inv_col<R, NT, TPB>), grid, dim3(TPB), 0, st, p);
+        return;
     }
+    if constexpr (R >= 64 && R <= 512) {
+        if (nt == 8) {
+            constexpr int TPB = SplitCfg<R, 8>::TPB;
+            dim3 grid(p.C / 8, slots);
+            if (kind == kInvCarry) hipLaunchKernelGGL((k_inv_carry<R, 8, TPB>), grid, dim3(TPB), 0, st, p);
+            else hipLaunchKernelGGL((k_adjust_fwd<R, 8, TPB>), grid, dim3(TPB), 0, st, p);
+            return;
+        }
+        if (nt == 16) {
+            constexpr int TPB = SplitCfg<R, 16>::TPB;
+            dim3 grid(p.C / 16, slots);
+            if (kind == kInvCarry) hipLaunchKernelGGL((k_inv_carry<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
+            else hipLaunchKernelGGL((k_adjust_fwd<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
+            return;
+        }
+    }
+    throw std::runtime_error("no split outer kernels for this column length");
 }
 
 template <int C>
-static void launch_row(const Params& p, hipStream_t st)
+static void launch_row(const Params& p, uint32_t slots, hipStream_t st)
 {
     constexpr int NT = RowCfg<C>::NT, TPB = RowCfg<C>::TPB;
-    dim3 grid(p.R / NT);
+    dim3 grid(p.R / NT, slots);
     hipLaunchKernelGGL((k_row<C, NT, TPB>), grid, dim3(TPB), 0, st, p);
 }
 
-static void dispatch_col(Kind kind, const Params& p, hipStream_t st)
+static void dispatch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, hipStream_t st)
 {
     switch (p.R) {
-    case 32: launch_col<32>(kind, p, st); break;
-    case 64: launch_col<64>(kind, p, st); break;
-    case 128: launch_col<128>(kind, p, st); break;
-    case 256: launch_col<256>(kind, p, st); break;
-    case 512: launch_col<512>(kind, p, st); break;
-    case 1024: launch_col<1024>(kind, p, st); break;
+    case 32: launch_col<32>(kind, p, slots, nt, st); break;
+    case 64: launch_col<64>(kind, p, slots, nt, st); break;
+    case 128: launch_col<128>(kind, p, slots, nt, st); break;
+    case 256: launch_col<256>(kind, p, slots, nt, st); break;
+    case 512: launch_col<512>(kind, p, slots, nt, st); break;
+    case 1024: launch_col<1024>(kind, p, slots, nt, st); break;
     default: throw std::runtime_error("unsupported column length");
     }
 }
 
-static void dispatch_row(const Params& p, hipStream_t st)
+static void dispatch_row(const Params& p, uint32_t slots, hipStream_t st)
 {
     switch (p.C) {
-    case 32: launch_row<32>(p, st); break;
-    case 64: launch_row<64>(p, st); break;
-    case 128: launch_row<128>(p, st); break;
-    case 256: launch_row<256>(p, st); break;
-    case 512: launch_row<512>(p, st); break;
-    case 1024: launch_row<1024>(p, st); break;
-    case 2048: launch_row<2048>(p, st); break;
-    case 4096: launch_row<4096>(p, st); break;
+    case 32: launch_row<32>(p, slots, st); break;
+    case 64: launch_row<64>(p, slots, st); break;
+    case 128: launch_row<128>(p, slots, st); break;
+    case 256: launch_row<256>(p, slots, st); break;
+    case 512: launch_row<512>(p, slots, st); break;
+    case 1024: launch_row<1024>(p, slots, st); break;
+    case 2048: launch_row<2048>(p, slots, st); break;
+    case 4096: launch_row<4096>(p, slots, st); break;
     default: throw std::runtime_error("unsupported row length");
     }
 }
 
 template <int RUN, int TPB>
-static void launch_carry_fixup(const Params& p, unsigned blocks, bool carry, bool fixup, hipStream_t st)
+static void launch_carry_fixup(const Params& p, unsigned blocks, uint32_t slots, bool carry, bool fixup, hipStream_t st)
 {
-    if (carry) hipLaunchKernelGGL((k_carry<RUN, TPB>), dim3(blocks), dim3(TPB), 0, st, p);
-    if (fixup) hipLaunchKernelGGL((k_fixup<RUN, TPB>), dim3((blocks + 63) / 64), dim3(64), 0, st, p, blocks);
+    if (carry) hipLaunchKernelGGL((k_carry<RUN, TPB>), dim3(blocks, slots), dim3(TPB), 0, st, p);
+    if (fixup)
+        hipLaunchKernelGGL((k_fixup<RUN, TPB>), dim3((blocks + 63) / 64, slots), dim3(64), 0, st, p, blocks);
 }
 
 // Correctly rounded tables from long double (64-bit mantissa) arithmetic.
@@ -770,32 +848,41 @@ static void host_tables(const Plan& pl, std::vector<double2>& tw, std::vector<do
     }
 }
 
-Engine::Engine(const Plan& p, int device, bool allow_fused) : plan_(p), device_(device)
+static uint32_t carry_tpb_for(const Plan& p) { return p.L >= p.run * 256u * 16u ? 256 : 64; }
+
+size_t Group::slot_bytes(const Plan& p)
 {
+    // residue (16 B/pt) + split-pass digits (4 B/pt) + run carries (24 B per
+    // run) + flat-carry block carries + scalars
+    const size_t runs = p.nt ? p.L / p.nt : 0;
+    const uint32_t tpb = carry_tpb_for(p);
+    return (size_t)p.L * 16 + (size_t)p.L * 4 + runs * 24 + (size_t)(p.L / (p.run * tpb)) * 24 + 64 + kChunk * 8;
+}
+
+Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capacity), device_(device)
+{
+    if (capacity == 0) throw std::invalid_argument("group capacity must be positive");
     check(hipSetDevice(device), "hipSetDevice");
     check(hipStreamCreate(&stream_), "hipStreamCreate");
     std::vector<double2> tw, twl_lo, twl_hi, om_lo, om_hi;
     std::vector<double> pw_lo, pw_hi, ipw_lo, ipw_hi;
     host_tables(p, tw, twl_lo, twl_hi, om_lo, om_hi, pw_lo, pw_hi, ipw_lo, ipw_hi);
-    run_ = ripple_digits(p) <= 8 ? 8 : 16;
-    carry_tpb_ = p.L >= (uint32_t)run_ * 256u * 16u ? 256 : 64;
-    carry_blocks_ = p.L / (run_ * carry_tpb_);
-
-    // The split outer pass needs kernels for this column length and runs
-    // (NT digits) long enough for any carry-in to settle.
-    plan_.fused = allow_fused && ripple_digits(p) <= kOuterNT && p.R >= 64 && p.R <= 1024;
-    const size_t runs = plan_.fused ? (size_t)p.L / kOuterNT : 0;
+    carry_tpb_ = carry_tpb_for(p);
+    carry_blocks_ = p.L / (p.run * carry_tpb_);
+    const size_t runs = p.nt ? (size_t)p.L / p.nt : 0;
 
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t off = 0;
-    const size_t o_z = off; off += al(sizeof(double2) * p.L);
-    const size_t o_car = off; off += al(sizeof(long long) * 3 * carry_blocks_);
-    const size_t o_runc = off; off += al(sizeof(long long) * 3 * runs);
-    const size_t o_rdig = off; off += al(sizeof(int) * kOuterNT * runs);
-    const size_t o_xhi = off; off += al(sizeof(long long));
-    const size_t o_err = off; off += al(sizeof(unsigned long long) * kGraphChunk);
+    const size_t o_z = off; off += al(sizeof(double2) * p.L * cap_);
+    const size_t o_car = off; off += al(sizeof(long long) * 3 * carry_blocks_ * cap_);
+    const size_t o_runc = off; off += al(sizeof(long long) * 3 * runs * cap_);
+    const size_t o_rdig = off; off += al(p.nt ? sizeof(int) * (size_t)p.L * cap_ : 0);
+    const size_t o_xhi = off; off += al(sizeof(long long) * cap_);
+    const size_t o_err = off; off += al(sizeof(unsigned long long) * kChunk * cap_);
     const size_t o_st = off; off += al(sizeof(int));
     const size_t o_dig = off; off += al(sizeof(int32_t) * p.L);
+    const size_t o_desc = off; off += al(sizeof(SlotDesc) * cap_);
+    const size_t o_state = off; off += al(sizeof(SlotState) * cap_);
     const size_t zero_end = off;
     const size_t o_tw = off; off += al(sizeof(double2) * tw.size());
     const size_t o_tlo = off; off += al(sizeof(double2) * twl_lo.size());
@@ -817,6 +904,8 @@ Engine::Engine(const Plan& p, int device, bool allow_fused) : plan_(p), device_(
     err_ = reinterpret_cast<unsigned long long*>(b + o_err);
     status_ = reinterpret_cast<int*>(b + o_st);
     digits_dev_ = reinterpret_cast<int32_t*>(b + o_dig);
+    desc_ = reinterpret_cast<SlotDesc*>(b + o_desc);
+    state_ = reinterpret_cast<SlotState*>(b + o_state);
     tabs_ = new DeviceTables;
     tabs_->tw = reinterpret_cast<double2*>(b + o_tw);
     tabs_->twl_lo = reinterpret_cast<double2*>(b + o_tlo);
@@ -841,160 +930,320 @@ Engine::Engine(const Plan& p, int device, bool allow_fused) : plan_(p), device_(
     up(tabs_->ipw_hi, ipw_hi.data(), sizeof(double) * ipw_hi.size());
     check(hipMemsetAsync(mem_, 0, zero_end, stream_), "hipMemsetAsync");
     check(hipStreamSynchronize(stream_), "init sync");
-    err_host_.resize(kGraphChunk);
+    desc_host_.assign(cap_, SlotDesc{ p.n, (long long)p.k, 0, 0, 0 });
+    check(hipMemcpy(desc_, desc_host_.data(), sizeof(SlotDesc) * cap_, hipMemcpyHostToDevice), "desc upload");
+    err_host_.resize((size_t)kChunk * cap_);
 }
 
-Engine::~Engine()
+Group::~Group()
 {
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
+    if (plain_exec_) (void)hipGraphExecDestroy(plain_exec_);
+    if (plain_graph_) (void)hipGraphDestroy(plain_graph_);
+    if (step_exec_) (void)hipGraphExecDestroy(step_exec_);
+    if (step_graph_) (void)hipGraphDestroy(step_graph_);
     if (mem_) (void)hipFree(mem_);
     if (stream_) (void)hipStreamDestroy(stream_);
     delete tabs_;
 }
 
-void Engine::carry(const Params& p, bool do_carry, bool do_fixup)
+Params Group::params(uint32_t slot0, bool check_active, uint32_t err_col) const
 {
-    if (run_ == 8 && carry_tpb_ == 256) launch_carry_fixup<8, 256>(p, carry_blocks_, do_carry, do_fixup, stream_);
-    else if (run_ == 8) launch_carry_fixup<8, 64>(p, carry_blocks_, do_carry, do_fixup, stream_);
-    else if (carry_tpb_ == 256) launch_carry_fixup<16, 256>(p, carry_blocks_, do_carry, do_fixup, stream_);
-    else launch_carry_fixup<16, 64>(p, carry_blocks_, do_carry, do_fixup, stream_);
+    Params p;
+    p.z = z_;
+    p.tw = tabs_->tw;
+    p.twl_lo = tabs_->twl_lo;
+    p.twl_hi = tabs_->twl_hi;
+    p.pw_lo = tabs_->pw_lo;
+    p.pw_hi = tabs_->pw_hi;
+    p.ipw_lo = tabs_->ipw_lo;
+    p.ipw_hi = tabs_->ipw_hi;
+    p.om_lo = tabs_->om_lo;
+    p.om_hi = tabs_->om_hi;
+    p.carries = carries_;
+    p.runc = runc_;
+    p.dig = rdig_;
+    p.xhi = xhi_;
+    p.err = err_ + err_col;
+    p.status = status_;
+    p.desc = desc_;
+    p.state = state_;
+    p.carr_stride = 3 * (size_t)carry_blocks_;
+    p.runc_stride = plan_.nt ? 3 * (size_t)(plan_.L / plan_.nt) : 0;
+    p.err_stride = kChunk;
+    p.slot0 = slot0;
+    p.sched = check_active ? 1 : 0;
+    p.n = plan_.n;
+    p.k = (long long)plan_.k;
+    p.mult = 1;
+    p.active = 1;
+    p.L = plan_.L;
+    p.logL = plan_.logL;
+    p.R = plan_.R;
+    p.C = plan_.C;
+    p.s = plan_.s;
+    p.smask = (1u << plan_.s) - 1;
+    return p;
 }
 
-Params Engine::params(uint64_t slot) const
+// One kernel kind over `slots` slots starting at p.slot0.
+void Group::launch(int kind, const Params& p, uint32_t slots)
 {
-    return make_params(plan_, z_, tabs_, carries_, runc_, rdig_, xhi_, err_ + slot, status_);
+    switch (kind) {
+    case kRow: dispatch_row(p, slots, stream_); break;
+    case kCarry:
+    case kFixup: {
+        const bool c = kind == kCarry, f = kind == kFixup;
+        if (plan_.run == 8 && carry_tpb_ == 256) launch_carry_fixup<8, 256>(p, carry_blocks_, slots, c, f, stream_);
+        else if (plan_.run == 8) launch_carry_fixup<8, 64>(p, carry_blocks_, slots, c, f, stream_);
+        else if (carry_tpb_ == 256) launch_carry_fixup<16, 256>(p, carry_blocks_, slots, c, f, stream_);
+        else launch_carry_fixup<16, 64>(p, carry_blocks_, slots, c, f, stream_);
+        break;
+    }
+    case kAdvance:
+        hipLaunchKernelGGL(k_advance, dim3((cap_ + 63) / 64), dim3(64), 0, stream_, state_, desc_, cap_);
+        break;
+    default: dispatch_col(kind, p, slots, plan_.nt, stream_);
+    }
 }
 
-void Engine::upload(const std::vector<int32_t>& digits, int64_t xhi)
+void Group::set_slot(uint32_t slot, uint32_t n, uint64_t k, uint32_t a, uint64_t total)
+{
+    if (slot >= cap_) throw std::invalid_argument("slot out of range");
+    if (n >> plan_.logL < 8 || (double)n / plan_.L > max_bits_per_digit(plan_.logL) + 1.0)
+        throw std::invalid_argument("n does not suit this group's transform length");
+    check(hipSetDevice(device_), "hipSetDevice");
+    unsigned top = 0;
+    while (top < 63 && (k >> (top + 1))) ++top;
+    desc_host_[slot] = SlotDesc{ n, (long long)k, a, top, total };
+    check(hipMemcpyAsync(desc_ + slot, &desc_host_[slot], sizeof(SlotDesc), hipMemcpyHostToDevice, stream_),
+          "desc upload");
+    check(hipStreamSynchronize(stream_), "desc sync");
+}
+
+void Group::set_state(uint32_t slot, uint64_t carried, bool active)
+{
+    SlotState s{ carried, active ? 1 : 0, 0 };
+    check(hipMemcpyAsync(state_ + slot, &s, sizeof s, hipMemcpyHostToDevice, stream_), "state upload");
+    check(hipStreamSynchronize(stream_), "state sync");
+}
+
+void Group::upload(uint32_t slot, const std::vector<int32_t>& digits, int64_t xhi)
 {
     check(hipSetDevice(device_), "hipSetDevice");
     if (digits.size() != plan_.L) throw std::runtime_error("digit count mismatch");
     long long xh = xhi;
     check(hipMemcpyAsync(digits_dev_, digits.data(), sizeof(int32_t) * plan_.L, hipMemcpyHostToDevice, stream_),
           "upload digits");
-    check(hipMemcpyAsync(xhi_, &xh, sizeof(long long), hipMemcpyHostToDevice, stream_), "upload xhi");
-    Params p = params(0);
-    hipLaunchKernelGGL(k_set_digits, dim3((plan_.L + 255) / 256), dim3(256), 0, stream_, p, digits_dev_);
+    check(hipMemcpyAsync(xhi_ + slot, &xh, sizeof(long long), hipMemcpyHostToDevice, stream_), "upload xhi");
+    Params p = params(slot, false, 0);
+    hipLaunchKernelGGL(k_set_digits, dim3((plan_.L + 255) / 256, 1), dim3(256), 0, stream_, p, digits_dev_);
     check(hipGetLastError(), "k_set_digits");
     check(hipStreamSynchronize(stream_), "upload sync");
 }
 
-void Engine::download(std::vector<int32_t>& digits, int64_t& xhi)
+void Group::download(uint32_t slot, std::vector<int32_t>& digits, int64_t& xhi)
 {
     check(hipSetDevice(device_), "hipSetDevice");
-    Params p = params(0);
-    hipLaunchKernelGGL(k_get_digits, dim3((plan_.L + 255) / 256), dim3(256), 0, stream_, p, digits_dev_);
+    Params p = params(slot, false, 0);
+    hipLaunchKernelGGL(k_get_digits, dim3((plan_.L + 255) / 256, 1), dim3(256), 0, stream_, p, digits_dev_);
     check(hipGetLastError(), "k_get_digits");
     digits.resize(plan_.L);
     long long xh = 0;
     check(hipMemcpyAsync(digits.data(), digits_dev_, sizeof(int32_t) * plan_.L, hipMemcpyDeviceToHost, stream_),
           "download digits");
-    check(hipMemcpyAsync(&xh, xhi_, sizeof(long long), hipMemcpyDeviceToHost, stream_), "download xhi");
+    check(hipMemcpyAsync(&xh, xhi_ + slot, sizeof(long long), hipMemcpyDeviceToHost, stream_), "download xhi");
     check(hipStreamSynchronize(stream_), "download sync");
     xhi = xh;
 }
 
-// g squarings from the digit state back to the digit state; iteration i's
-// round-off lands in err slot i.
-void Engine::enqueue_chunk(int g)
+// g squarings of every slot from the digit state back to the digit state;
+// iteration i's round-off lands in err column i.
+void Group::enqueue_plain_chunk(int g)
 {
-    const Params p0 = params(0);
-    dispatch_col(Kind::FwdCol, p0, stream_);
-    dispatch_row(p0, stream_);
+    launch(kFwdCol, params(0, false, 0), cap_);
+    launch(kRow, params(0, false, 0), cap_);
     for (int i = 0; i + 1 < g; ++i) {
-        const Params p = params((uint64_t)i);
-        if (plan_.fused) {
-            dispatch_col(Kind::InvCarry, p, stream_);
-            dispatch_col(Kind::AdjFwd, p, stream_);
+        const Params p = params(0, false, (uint32_t)i);
+        if (plan_.nt) {
+            launch(kInvCarry, p, cap_);
+            launch(kAdjFwd, p, cap_);
         } else {
-            dispatch_col(Kind::InvCol, p, stream_);
-            carry(p, true, true);
-            dispatch_col(Kind::FwdCol, p, stream_);
+            launch(kInvCol, p, cap_);
+            launch(kCarry, p, cap_);
+            launch(kFixup, p, cap_);
+            launch(kFwdCol, p, cap_);
         }
-        dispatch_row(p, stream_);
+        launch(kRow, p, cap_);
     }
-    const Params pl = params((uint64_t)(g - 1));
-    dispatch_col(Kind::InvCol, pl, stream_);
-    carry(pl, true, true);
+    const Params pl = params(0, false, (uint32_t)(g - 1));
+    launch(kInvCol, pl, cap_);
+    launch(kCarry, pl, cap_);
+    launch(kFixup, pl, cap_);
 }
 
-void Engine::build_graph(int iters)
+// g scheduled steps of every active slot; step i's round-off in err column i.
+void Group::enqueue_step_chunk(int g)
 {
-    if (graph_exec_) {
-        (void)hipGraphExecDestroy(graph_exec_);
-        graph_exec_ = nullptr;
+    for (int i = 0; i < g; ++i) {
+        const Params p = params(0, true, (uint32_t)i);
+        if (plan_.nt) {
+            launch(kInvCarry, p, cap_);
+            launch(kAdjFwd, p, cap_);
+        } else {
+            launch(kInvCol, p, cap_);
+            launch(kCarry, p, cap_);
+            launch(kFixup, p, cap_);
+            launch(kFwdCol, p, cap_);
+        }
+        launch(kRow, p, cap_);
+        launch(kAdvance, p, cap_);
     }
-    if (graph_) {
-        (void)hipGraphDestroy(graph_);
-        graph_ = nullptr;
-    }
-    check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kGraphChunk), 0, stream_, err_, kGraphChunk);
-    enqueue_chunk(iters);
-    check(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
-    check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
-    graph_iters_ = iters;
 }
 
-double Engine::square(uint64_t iters, double* err_trace)
+void Group::build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g)
+{
+    check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    const int cnt = (int)(kChunk * cap_);
+    hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, err_, cnt);
+    if (stepping) enqueue_step_chunk(g);
+    else enqueue_plain_chunk(g);
+    check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+// Reads the chunk's err columns: trace gets slot 0's per-column values,
+// slot_err each slot's max; returns the max over everything.
+double Group::read_err(int g, double* trace, double* slot_err)
+{
+    check(hipMemcpyAsync(err_host_.data(), err_, sizeof(unsigned long long) * kChunk * cap_, hipMemcpyDeviceToHost,
+                         stream_),
+          "err D2H");
+    check(hipStreamSynchronize(stream_), "chunk sync");
+    double emax = 0.0;
+    for (uint32_t s = 0; s < cap_; ++s) {
+        double smax = 0.0;
+        for (int i = 0; i < g; ++i) {
+            double e;
+            std::memcpy(&e, &err_host_[(size_t)s * kChunk + i], sizeof e);
+            if (s == 0 && trace) trace[i] = e;
+            smax = std::max(smax, e);
+        }
+        if (slot_err) slot_err[s] = smax;
+        emax = std::max(emax, smax);
+    }
+    return emax;
+}
+
+double Group::square(uint64_t iters, double* err_trace)
 {
     check(hipSetDevice(device_), "hipSetDevice");
     double emax = 0.0;
     uint64_t done = 0;
     while (done < iters) {
-        const int g = (int)std::min<uint64_t>(kGraphChunk, iters - done);
-        if (g == kGraphChunk) {
-            if (!graph_exec_) build_graph(kGraphChunk);
-            check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+        const int g = (int)std::min<uint64_t>(kChunk, iters - done);
+        if (g == kChunk) {
+            if (!plain_exec_) build_graph(plain_exec_, plain_graph_, false, kChunk);
+            check(hipGraphLaunch(plain_exec_, stream_), "hipGraphLaunch");
         } else {
-            hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kGraphChunk), 0, stream_, err_, kGraphChunk);
-            enqueue_chunk(g);
+            const int cnt = (int)(kChunk * cap_);
+            hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, err_, cnt);
+            enqueue_plain_chunk(g);
             check(hipGetLastError(), "square launch");
         }
-        check(hipMemcpyAsync(err_host_.data(), err_, sizeof(unsigned long long) * g, hipMemcpyDeviceToHost, stream_),
-              "err D2H");
-        check(hipStreamSynchronize(stream_), "square sync");
-        for (int i = 0; i < g; ++i) {
-            double e;
-            std::memcpy(&e, &err_host_[i], sizeof e);
-            if (err_trace) err_trace[done + i] = e;
-            if (e > emax) emax = e;
-        }
+        emax = std::max(emax, read_err(g, err_trace ? err_trace + done : nullptr, nullptr));
         done += g;
     }
     return emax;
 }
 
-void Engine::kernel_times(int reps, double* ms)
+void Group::enter(uint32_t slot)
+{
+    check(hipSetDevice(device_), "hipSetDevice");
+    const Params p = params(slot, false, 0);
+    launch(kFwdCol, p, 1);
+    launch(kRow, p, 1);
+    check(hipGetLastError(), "enter");
+}
+
+double Group::step(uint32_t steps, double* slot_err)
+{
+    check(hipSetDevice(device_), "hipSetDevice");
+    double emax = 0.0;
+    std::vector<double> se(cap_, 0.0), tmp(cap_, 0.0);
+    uint32_t done = 0;
+    while (done < steps) {
+        const int g = (int)std::min<uint32_t>(kChunk, steps - done);
+        if (g == kChunk) {
+            if (!step_exec_) build_graph(step_exec_, step_graph_, true, kChunk);
+            check(hipGraphLaunch(step_exec_, stream_), "hipGraphLaunch");
+        } else {
+            const int cnt = (int)(kChunk * cap_);
+            hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, err_, cnt);
+            enqueue_step_chunk(g);
+            check(hipGetLastError(), "step launch");
+        }
+        emax = std::max(emax, read_err(g, nullptr, tmp.data()));
+        for (uint32_t s = 0; s < cap_; ++s) se[s] = std::max(se[s], tmp[s]);
+        done += g;
+    }
+    if (slot_err) std::copy(se.begin(), se.end(), slot_err);
+    return emax;
+}
+
+double Group::leave(uint32_t slot)
+{
+    check(hipSetDevice(device_), "hipSetDevice");
+    unsigned long long zero = 0;
+    check(hipMemcpyAsync(err_ + (size_t)slot * kChunk, &zero, sizeof zero, hipMemcpyHostToDevice, stream_), "err reset");
+    // the schedule multiplier of the last squaring comes from the slot's state
+    Params p = params(slot, true, 0);
+    p.sched = 1;
+    SlotState st{};
+    check(hipMemcpyAsync(&st, state_ + slot, sizeof st, hipMemcpyDeviceToHost, stream_), "state D2H");
+    check(hipStreamSynchronize(stream_), "state sync");
+    // the kernels skip inactive slots; the leave sequence runs regardless
+    SlotState on{ st.carried, 1, 0 };
+    check(hipMemcpyAsync(state_ + slot, &on, sizeof on, hipMemcpyHostToDevice, stream_), "state H2D");
+    launch(kInvCol, p, 1);
+    launch(kCarry, p, 1);
+    launch(kFixup, p, 1);
+    SlotState fin{ st.carried + 1, 0, 0 };
+    check(hipMemcpyAsync(state_ + slot, &fin, sizeof fin, hipMemcpyHostToDevice, stream_), "state H2D");
+    unsigned long long e = 0;
+    check(hipMemcpyAsync(&e, err_ + (size_t)slot * kChunk, sizeof e, hipMemcpyDeviceToHost, stream_), "err D2H");
+    check(hipStreamSynchronize(stream_), "leave sync");
+    double d;
+    std::memcpy(&d, &e, sizeof d);
+    return d;
+}
+
+std::vector<SlotState> Group::states()
+{
+    std::vector<SlotState> v(cap_);
+    check(hipMemcpyAsync(v.data(), state_, sizeof(SlotState) * cap_, hipMemcpyDeviceToHost, stream_), "state D2H");
+    check(hipStreamSynchronize(stream_), "state sync");
+    return v;
+}
+
+void Group::kernel_times(int reps, double* ms)
 {
     check(hipSetDevice(device_), "hipSetDevice");
     std::vector<int32_t> saved;
     int64_t saved_xhi = 0;
-    download(saved, saved_xhi);
-    const Params p = params(0);
+    download(0, saved, saved_xhi);
+    const Params p = params(0, false, 0);
     hipEvent_t a, b;
     check(hipEventCreate(&a), "hipEventCreate");
     check(hipEventCreate(&b), "hipEventCreate");
     for (int kind = 0; kind < kNumKernels; ++kind) {
-        if (kind >= 5 && !plan_.fused) {
+        if (kind >= kInvCarry && !plan_.nt) {
             ms[kind] = 0.0;
             continue;
         }
         for (int pass = 0; pass < 2; ++pass) {
             const int cnt = pass == 0 ? 1 : reps;
             if (pass == 1) check(hipEventRecord(a, stream_), "hipEventRecord");
-            for (int i = 0; i < cnt; ++i) {
-                switch (kind) {
-                case 0: dispatch_col(Kind::FwdCol, p, stream_); break;
-                case 1: dispatch_row(p, stream_); break;
-                case 2: dispatch_col(Kind::InvCol, p, stream_); break;
-                case 3: carry(p, true, false); break;
-                case 4: carry(p, false, true); break;
-                case 5: dispatch_col(Kind::InvCarry, p, stream_); break;
-                default: dispatch_col(Kind::AdjFwd, p, stream_); break;
-                }
-            }
+            for (int i = 0; i < cnt; ++i) launch(kind, p, cap_);
             if (pass == 1) check(hipEventRecord(b, stream_), "hipEventRecord");
         }
         check(hipEventSynchronize(b), "hipEventSynchronize");
@@ -1005,15 +1254,21 @@ void Engine::kernel_times(int reps, double* ms)
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     // the timing launches leave arbitrary data and status behind
-    check(hipMemsetAsync(status_, 0, sizeof(int), stream_), "status reset");
-    upload(saved, saved_xhi);
+    clear_status();
+    upload(0, saved, saved_xhi);
 }
 
-int Engine::status()
+int Group::status()
 {
     int s = 0;
     check(hipMemcpy(&s, status_, sizeof(int), hipMemcpyDeviceToHost), "status D2H");
     return s;
+}
+
+void Group::clear_status()
+{
+    check(hipMemsetAsync(status_, 0, sizeof(int), stream_), "status reset");
+    check(hipStreamSynchronize(stream_), "status sync");
 }
 
 }  // namespace pleng

@@ -1,0 +1,286 @@
+// pool.cpp -- continuous batching of Proth tests on one GPU (pl_pool_* in
+// include/prothcore.h).
+//
+// Candidates wait in a FIFO.  Admission places a candidate in a free slot of
+// the group for its transform length, creating that group when its whole
+// allocation fits in the remaining HBM budget.  Every run() call advances
+// all active slots of every group by scheduled steps (seed powering a^k,
+// then n-1 squarings, all on the device), then retires the slots whose tests
+// have finished and reports their verdicts.  Freed slots are refilled at the
+// next admission, so requests that arrive while others are on the GPU join
+// the running batch.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bignum.h"
+#include "engine.h"
+#include "prothcore.h"
+
+namespace plpool {
+
+using clk = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t k;
+    uint32_t n;
+    uint64_t tag;
+    uint64_t stop_after;
+    double t_submit;
+};
+
+struct Occupant {
+    bool used = false;
+    Request req{};
+    uint64_t base = 0;
+    uint64_t total = 0;
+    double t_admit = 0;
+    double max_err = 0;
+    std::unique_ptr<plbn::Modulus> M;
+};
+
+struct LenGroup {
+    pleng::Plan plan;
+    std::unique_ptr<pleng::Group> g;
+    std::vector<Occupant> slots;
+};
+
+}  // namespace plpool
+
+using namespace plpool;
+
+struct pl_pool {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t used_bytes = 0;
+    uint32_t max_slots = 0;
+    uint64_t k_max = 0;
+    std::map<uint32_t, std::unique_ptr<LenGroup>> groups;  // by log2 L
+    std::deque<Request> queue;
+    std::vector<pl_pool_result> done;
+    clk::time_point t0 = clk::now();
+    double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
+};
+
+namespace {
+thread_local std::string g_pool_err;
+int pfail(int code, const std::string& m)
+{
+    g_pool_err = m;
+    return code;
+}
+
+unsigned topbit64(uint64_t k)
+{
+    unsigned t = 0;
+    while (t < 63 && (k >> (t + 1))) ++t;
+    return t;
+}
+
+// Lowest bits per digit a slot of this length can see, and the layout for it.
+LenGroup* group_for(pl_pool* P, uint32_t n, uint64_t k, std::string& why)
+{
+    pleng::Plan pl;
+    if (!pleng::choose_plan(n, k, 0, pl, why)) return nullptr;
+    auto it = P->groups.find(pl.logL);
+    if (it != P->groups.end()) return it->second.get();
+    // n just above the previous length's limit gives the fewest bits per digit
+    const double bmin = pl.logL > 10 ? pleng::max_bits_per_digit(pl.logL - 1) / 2.0 : 8.0;
+    if (!pleng::choose_layout(pl, std::max(8.0, bmin), P->k_max, 100, true, why)) return nullptr;
+    const size_t per_slot = pleng::Group::slot_bytes(pl);
+    const uint64_t left = P->budget > P->used_bytes ? P->budget - P->used_bytes : 0;
+    uint32_t cap = (uint32_t)std::min<uint64_t>(P->max_slots, left / (per_slot + 1));
+    if (cap == 0) {
+        why = "HBM budget exhausted";
+        return nullptr;
+    }
+    auto lg = std::make_unique<LenGroup>();
+    lg->plan = pl;
+    lg->g = std::make_unique<pleng::Group>(pl, cap, P->device);
+    lg->slots.resize(cap);
+    P->used_bytes += lg->g->device_bytes();
+    LenGroup* raw = lg.get();
+    P->groups[pl.logL] = std::move(lg);
+    return raw;
+}
+
+void admit(pl_pool* P)
+{
+    for (auto it = P->queue.begin(); it != P->queue.end();) {
+        const Request r = *it;
+        std::string why;
+        LenGroup* lg = group_for(P, r.n, r.k, why);
+        if (!lg) {
+            ++it;  // no room for this length yet; later candidates may still fit
+            continue;
+        }
+        int free_slot = -1;
+        for (size_t s = 0; s < lg->slots.size(); ++s)
+            if (!lg->slots[s].used) {
+                free_slot = (int)s;
+                break;
+            }
+        if (free_slot < 0) {
+            ++it;
+            continue;
+        }
+        Occupant& o = lg->slots[free_slot];
+        o.used = true;
+        o.req = r;
+        o.base = pl_choose_base(r.k, r.n);
+        o.M = std::make_unique<plbn::Modulus>(r.n, r.k);
+        o.t_admit = P->now();
+        o.max_err = 0;
+        const unsigned top = topbit64(r.k);
+        const uint64_t sq = r.stop_after ? std::min<uint64_t>(r.stop_after, r.n - 1) : r.n - 1;
+        o.total = top + sq;
+        std::vector<int32_t> d;
+        int64_t xhi = 0;
+        plbn::to_digits(*o.M, lg->plan.L, plbn::to_mont(*o.M, plbn::Limbs(1, o.base ? o.base : 0)), d, xhi);
+        lg->g->set_slot((uint32_t)free_slot, r.n, r.k, (uint32_t)o.base, o.total);
+        lg->g->upload((uint32_t)free_slot, d, xhi);
+        lg->g->set_state((uint32_t)free_slot, 0, o.total > 1);
+        lg->g->enter((uint32_t)free_slot);
+        it = P->queue.erase(it);
+    }
+}
+
+void retire(pl_pool* P)
+{
+    for (auto& kv : P->groups) {
+        LenGroup& lg = *kv.second;
+        const auto st = lg.g->states();
+        for (size_t s = 0; s < lg.slots.size(); ++s) {
+            Occupant& o = lg.slots[s];
+            if (!o.used || st[s].active) continue;
+            o.max_err = std::max(o.max_err, lg.g->leave((uint32_t)s));
+            std::vector<int32_t> d;
+            int64_t xhi = 0;
+            lg.g->download((uint32_t)s, d, xhi);
+            const plbn::Limbs x = plbn::from_mont(*o.M, plbn::from_digits(*o.M, lg.plan.L, d, xhi));
+            pl_pool_result r{};
+            r.tag = o.req.tag;
+            r.k = o.req.k;
+            r.n = o.req.n;
+            r.fft_len = lg.plan.L;
+            r.base = o.base;
+            r.res64 = plbn::low64(x);
+            r.max_err = o.max_err;
+            r.squarings = o.total;
+            const bool full = !o.req.stop_after || o.req.stop_after >= (uint64_t)o.req.n - 1;
+            r.verdict = full ? (plbn::is_n_minus_1(*o.M, x) ? 1 : 0) : -1;
+            r.gated = 0;
+            r.t_submit = o.req.t_submit;
+            r.t_admit = o.t_admit;
+            r.t_done = P->now();
+            P->done.push_back(r);
+            o.used = false;
+            o.M.reset();
+        }
+    }
+}
+
+size_t active_count(pl_pool* P)
+{
+    size_t a = 0;
+    for (auto& kv : P->groups)
+        for (auto& o : kv.second->slots) a += o.used ? 1 : 0;
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pl_pool_last_error(void) { return g_pool_err.c_str(); }
+
+int pl_pool_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots_per_length, uint64_t k_max,
+                   pl_pool** out)
+{
+    if (!out) return pfail(PL_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return pfail(PL_ERR_HIP, "no HIP device visible");
+    if (device < 0 || device >= nd) return pfail(PL_ERR_ARG, "device index out of range");
+    auto P = std::make_unique<pl_pool>();
+    P->device = device;
+    P->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
+    P->max_slots = max_slots_per_length ? max_slots_per_length : 256;
+    P->k_max = k_max ? k_max : (1ull << 32) - 1;
+    *out = P.release();
+    return PL_OK;
+}
+
+void pl_pool_destroy(pl_pool* P) { delete P; }
+
+int pl_pool_submit(pl_pool* P, uint64_t k, uint32_t n, uint64_t tag, uint64_t stop_after)
+{
+    if (!P) return pfail(PL_ERR_ARG, "NULL pool");
+    if (k == 0 || (k & 1) == 0 || k > P->k_max) return pfail(PL_ERR_ARG, "k must be odd and at most k_max");
+    pleng::Plan pl;
+    std::string why;
+    if (!pleng::choose_plan(n, k, 0, pl, why)) return pfail(PL_ERR_ARG, why);
+    const double t = P->now();
+    if (pl_gate_rejects(k, n)) {
+        pl_pool_result r{};
+        r.tag = tag;
+        r.k = k;
+        r.n = n;
+        r.gated = 1;
+        r.verdict = 0;
+        r.t_submit = r.t_admit = r.t_done = t;
+        P->done.push_back(r);
+        return PL_OK;
+    }
+    P->queue.push_back(Request{ k, n, tag, stop_after, t });
+    return PL_OK;
+}
+
+int pl_pool_run(pl_pool* P, uint32_t steps, pl_pool_result* out, size_t cap, size_t* n_out)
+{
+    if (!P || (!out && cap) || !n_out) return pfail(PL_ERR_ARG, "NULL argument");
+    try {
+        admit(P);
+        for (auto& kv : P->groups) {
+            LenGroup& lg = *kv.second;
+            bool any = false;
+            for (auto& o : lg.slots) any = any || o.used;
+            if (!any) continue;
+            std::vector<double> se(lg.slots.size(), 0.0);
+            lg.g->step(steps, se.data());
+            if (lg.g->status()) {
+                lg.g->clear_status();
+                return pfail(PL_ERR_STATE, "device carry did not settle");
+            }
+            for (size_t s = 0; s < lg.slots.size(); ++s)
+                if (lg.slots[s].used) lg.slots[s].max_err = std::max(lg.slots[s].max_err, se[s]);
+        }
+        retire(P);
+        admit(P);
+    } catch (const std::exception& e) {
+        return pfail(PL_ERR_HIP, e.what());
+    }
+    const size_t m = std::min(cap, P->done.size());
+    for (size_t i = 0; i < m; ++i) out[i] = P->done[i];
+    P->done.erase(P->done.begin(), P->done.begin() + (long)m);
+    *n_out = m;
+    return PL_OK;
+}
+
+int pl_pool_status(pl_pool* P, size_t* queued, size_t* active, size_t* finished_unread, uint64_t* hbm_bytes)
+{
+    if (!P) return pfail(PL_ERR_ARG, "NULL pool");
+    if (queued) *queued = P->queue.size();
+    if (active) *active = active_count(P);
+    if (finished_unread) *finished_unread = P->done.size();
+    if (hbm_bytes) *hbm_bytes = P->used_bytes;
+    return PL_OK;
+}
+
+}  // extern "C"
