@@ -148,6 +148,116 @@ def cpu_baseline(k, n, budget_s=12.0):
                        f"at L={L} (the shortest length whose limb layout holds the residue), single thread")
 
 
+def _topbit(k: int) -> int:
+    return k.bit_length() - 1
+
+
+def plain_per_squaring(core, k, n, iters=1500):
+    """Steady-state seconds per squaring of one candidate alone on the GPU
+    (the plain one-candidate-at-a-time loop: one context, graph-captured)."""
+    c = core.Context(k, n)
+    c.set_residue(core.choose_base(k, n) ** 3)
+    c.square(64)
+    t0 = time.perf_counter()
+    c.square(iters)
+    dt = time.perf_counter() - t0
+    L = c.fft_len
+    c.close()
+    return dt / iters, L
+
+
+def batch_c4(core, workloads, steps=1000):
+    """Config 4: 256 candidates, n uniform in [2^18, 2^20], advanced together
+    by the pool (one launch per kernel kind per length group per squaring
+    step).  Each test is run for its seed powering plus `steps` squarings;
+    tests/hour extrapolates the measured squaring throughput to full tests
+    (topbit(k) + n - 1 squarings each).  The plain loop runs one candidate at
+    a time: its per-squaring time is measured per transform length."""
+    w = workloads.c4()
+    cands = w.candidates
+    full = [_topbit(k) + n - 1 for k, n in cands]
+    with core.Pool(k_max=1 << 16) as pool:
+        t0 = time.perf_counter()
+        for i, (k, n) in enumerate(cands):
+            pool.submit(k, n, tag=i, stop_after=steps)
+        res = pool.drain(steps=64)
+        t1 = time.perf_counter()
+    done_sq = sum(r["squarings"] for r in res)
+    thr = done_sq / (t1 - t0)
+    batched_tph = 3600.0 * len(cands) / (sum(full) / thr)
+    per_len = {}
+    for (k, n) in cands:
+        L = core.plan_fft_len(n, k)
+        if L not in per_len:
+            per_len[L] = plain_per_squaring(core, k, n)[0]
+    plain_time = sum(per_len[core.plan_fft_len(n, k)] * f for (k, n), f in zip(cands, full))
+    plain_tph = 3600.0 * len(cands) / plain_time
+    return dict(workload="c4_batch: 256 candidates, k odd in [3,2^16), n uniform in [2^18,2^20] (PCG64 seed 4)",
+                gate_redraws=w.redraws, measured_squarings=done_sq, wall_s=round(t1 - t0, 3),
+                squarings_per_s=round(thr, 1), batched_tests_per_hour=round(batched_tph, 1),
+                plain_tests_per_hour=round(plain_tph, 1), speedup_vs_plain=round(batched_tph / plain_tph, 2),
+                plain_us_per_squaring_by_fft_len={str(L): round(v * 1e6, 2) for L, v in sorted(per_len.items())},
+                max_err=max(r["max_err"] for r in res),
+                note=f"each test ran its seed powering + {steps} squarings; tests/hour extrapolates to n-1")
+
+
+def serve_c5(core, workloads, count=200, scale=100, load=2.0, seed=5):
+    """Config 5 (scaled): Poisson arrivals of the first `count` config-5
+    candidates (k odd in [3,2^20), n log-uniform in [5e4,4e6]), each test cut
+    to (n-1)/scale squarings after its seed so the run stays bounded.
+    Arrival rate = `load` x the plain sequential loop's capacity.  The pool
+    admits arrivals as slots free up (continuous batching); the plain loop's
+    latencies come from a FIFO single-server replay of the same arrivals with
+    its measured per-squaring times."""
+    import numpy as np
+    w = workloads.c5(count)
+    cands = w.candidates
+    sq = [_topbit(k) + max(1, (n - 1) // scale) for k, n in cands]
+    per_len = {}
+    for (k, n) in cands:
+        L = core.plan_fft_len(n, k)
+        if L not in per_len:
+            per_len[L] = plain_per_squaring(core, k, n, iters=800)[0]
+    svc = [per_len[core.plan_fft_len(n, k)] * s for (k, n), s in zip(cands, sq)]
+    lam = load / (sum(svc) / len(svc))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(cands)))
+    lat = {}
+    with core.Pool(k_max=1 << 20) as pool:
+        t0 = time.perf_counter()
+        nxt = 0
+        while len(lat) < len(cands):
+            now = time.perf_counter() - t0
+            while nxt < len(cands) and arrivals[nxt] <= now:
+                k, n = cands[nxt]
+                pool.submit(k, n, tag=nxt, stop_after=max(1, (n - 1) // scale))
+                nxt += 1
+            st = pool.status()
+            if st["active"] == 0 and st["queued"] == 0 and nxt < len(cands):
+                time.sleep(max(0.0, min(0.01, arrivals[nxt] - now)))
+            for r in pool.run(steps=64):
+                lat[r["tag"]] = r["t_done"] - r["t_submit"]
+        makespan = time.perf_counter() - t0
+    # plain sequential loop on the same arrivals (FIFO, one test at a time)
+    free_at, plat = 0.0, []
+    for a, s in zip(arrivals, svc):
+        start = max(a, free_at)
+        free_at = start + s
+        plat.append(free_at - a)
+    L_b = np.array([lat[i] for i in range(len(cands))])
+    L_p = np.array(plat)
+    return dict(workload=f"c5_serving (scaled): first {count} config-5 candidates, each cut to (n-1)/{scale} "
+                         "squarings after the seed, Poisson arrivals",
+                arrival_rate_per_s=round(lam, 3), offered_load_vs_plain_capacity=load,
+                p50_time_to_verdict_s=round(float(np.percentile(L_b, 50)), 4),
+                p99_time_to_verdict_s=round(float(np.percentile(L_b, 99)), 4),
+                plain_p50_s=round(float(np.percentile(L_p, 50)), 4),
+                plain_p99_s=round(float(np.percentile(L_p, 99)), 4),
+                tests_per_hour=round(3600.0 * len(cands) / makespan, 1),
+                plain_tests_per_hour=round(3600.0 * len(cands) / max(free_at, 1e-9), 1),
+                makespan_s=round(makespan, 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +265,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the n=2^24 line")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (config 4) leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every configuration fits one MI355X and candidates are "
@@ -194,6 +306,15 @@ def main():
                               kernels_per_squaring=r3["kernels_per_squaring"], bytes_per_squaring=sq3_bytes,
                               hbm_roofline_us=round(roofline_us, 3), x_of_roofline=round(us3 / roofline_us, 2))
         log(f"[bench] c3: {us3:.2f} us/squaring, max_err={r3['max_err']:.4g}")
+
+    if not args.no_batch:
+        log("[bench] c4 batch")
+        extra["batch_c4"] = batch_c4(core, workloads)
+        log(f"[bench] c4: {extra['batch_c4']}")
+    if not args.no_serve:
+        log("[bench] c5 serving")
+        extra["serve_c5"] = serve_c5(core, workloads)
+        log(f"[bench] c5: {extra['serve_c5']}")
 
     cpu = None
     if not args.no_cpu:
