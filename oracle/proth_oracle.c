@@ -635,7 +635,9 @@ static void add_limbs(uint64_t* a, const uint64_t* b, size_t nl) /* a += b */
  * v == Lo + r*2^n - q (mod N) (k*2^n == -1); q is reduced recursively. */
 void or_mod_N(uint64_t* v, size_t nl, unsigned n, uint64_t k)
 {
-    size_t wl = nl + 2;
+    /* Working width must hold N itself, even when v has fewer limbs. */
+    const size_t n_of_N = (size_t)n / 64 + 2;
+    size_t wl = (nl > n_of_N ? nl : n_of_N) + 2;
     uint64_t* Nl = (uint64_t*)calloc(wl, sizeof(uint64_t));
     make_N(Nl, wl, n, k);
     uint64_t* w = (uint64_t*)calloc(wl, sizeof(uint64_t));
