@@ -103,12 +103,18 @@ def roofline(kt: dict, L: int, nt: int):
 
 
 def measured_traffic(kernel: str, workload: str):
-    p = ROOT / "profiles" / f"traffic_{kernel}_{workload}.json"
-    if p.exists():
-        try:
-            return json.loads(p.read_text()).get("bytes_per_launch")
-        except Exception:
-            return None
+    """HBM bytes per launch of `kernel`, from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a copy kernel of known
+    size); None when no summary exists for this kernel and workload."""
+    p = ROOT / "profiles" / f"traffic_{kernel}_{workload}.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("bytes_per_launch:"):
+            try:
+                return int(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
     return None
 
 
@@ -120,7 +126,6 @@ def cpu_baseline(k, n, budget_s=12.0):
     from oracle import Oracle, RefLib, int_to_limbs
     import random
     o = Oracle()
-    L = o.auto_tune_fft_len.__self__ and None
     L = 16
     while not o.fits(L, n, k) or n / L > 18.0:
         L *= 2
@@ -254,7 +259,7 @@ def serve_c5(core, workloads, count=200, scale=100, load=2.0, seed=5):
                 plain_p50_s=round(float(np.percentile(L_p, 50)), 4),
                 plain_p99_s=round(float(np.percentile(L_p, 99)), 4),
                 tests_per_hour=round(3600.0 * len(cands) / makespan, 1),
-                plain_tests_per_hour=round(3600.0 * len(cands) / max(free_at, 1e-9), 1),
+                plain_tests_per_hour=round(3600.0 * len(cands) / max(float(free_at), 1e-9), 1),
                 makespan_s=round(makespan, 3))
 
 

@@ -99,7 +99,8 @@ int pl_fft_square_karatsuba(pl_ctx* ctx, uint64_t* v, size_t n_limbs, unsigned n
 
 /* Diagnostics (no counterpart in the reference): average duration in ms of
  * each kernel of one squaring, timed with HIP events on the context's own
- * stream over `reps` launches, into ms[0..6]: ms[0] forward column pass,
+ * stream between consecutive kernels of `reps` real squarings run in their
+ * steady-state order, into ms[0..6]: ms[0] forward column pass,
  * ms[1] row pass (forward DFT, pointwise square, inverse DFT), ms[2] inverse
  * column pass, ms[3] flat carry/fold, ms[4] block fix-up, ms[5] inverse
  * column DFT + carry/fold, ms[6] carry-in + forward column DFT (ms[5] and
@@ -108,6 +109,12 @@ int pl_fft_square_karatsuba(pl_ctx* ctx, uint64_t* v, size_t n_limbs, unsigned n
  * *traffic_bytes (nullable) gets the algorithmic HBM bytes one launch of
  * each full-array kernel moves (read + write of the L complex points). */
 int pl_kernel_times(pl_ctx* ctx, int reps, double* ms, uint64_t* traffic_bytes);
+
+/* Diagnostics (no counterpart in the reference): *ms <- average duration of
+ * a plain device-to-device copy of `bytes` bytes (reads and writes `bytes`
+ * each) over `reps` launches on `device`.  The measured streaming rate of
+ * the card, and the known-size kernel that calibrates byte counters. */
+int pl_copy_probe(int device, uint64_t bytes, int reps, double* ms);
 
 /* Small-prime gate (P/proth_logic.cpp:41-50, 405-408): 1 when some odd
  * prime p <= 67 divides k*2^n+1. */

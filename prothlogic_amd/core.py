@@ -82,6 +82,7 @@ def lib() -> ctypes.CDLL:
         "pl_is_n_minus_1": ([vp, ctypes.POINTER(i32)], i32),
         "pl_fft_square_karatsuba": ([vp, _u64p, sz, ctypes.c_uint, u64, _dp], i32),
         "pl_kernel_times": ([vp, i32, _dp, _u64p], i32),
+        "pl_copy_probe": ([i32, u64, i32, _dp], i32),
         "pl_gate_rejects": ([u64, u32], i32),
         "pl_choose_base": ([u64, u32], u64),
         "pl_proth_test": ([ctypes.POINTER(PlTestArgs), ctypes.POINTER(PlTestResult)], i32),
@@ -136,6 +137,13 @@ def gate_rejects(k: int, n: int) -> bool:
 
 def choose_base(k: int, n: int) -> int:
     return int(lib().pl_choose_base(k, n))
+
+
+def copy_probe(nbytes: int, reps: int = 10, device: int = 0) -> float:
+    """ms per plain device-to-device copy of nbytes (reads and writes nbytes)."""
+    ms = ctypes.c_double(0.0)
+    _check(lib().pl_copy_probe(device, nbytes, reps, ctypes.byref(ms)))
+    return ms.value
 
 
 class Context:

@@ -269,6 +269,18 @@ int pl_kernel_times(pl_ctx* ctx, int reps, double* ms, uint64_t* traffic_bytes)
     });
 }
 
+int pl_copy_probe(int device, uint64_t bytes, int reps, double* ms)
+{
+    if (!ms || bytes < 16 || reps <= 0) return fail(PL_ERR_ARG, "bad argument");
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return fail(PL_ERR_HIP, "no HIP device visible");
+    if (device < 0 || device >= nd) return fail(PL_ERR_ARG, "device index out of range");
+    return guarded([&] {
+        *ms = pleng::copy_probe(device, (size_t)bytes, reps);
+        return PL_OK;
+    });
+}
+
 int pl_gate_rejects(uint64_t k, uint32_t n)
 {
     static const uint64_t ps[] = { 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67 };

@@ -96,11 +96,14 @@ class Group {
     void clear_status();
 
     // Average duration (ms) of each kernel over all slots, timed with HIP
-    // events on this group's stream over `reps` launches: [0] forward
-    // column pass, [1] row pass, [2] inverse column pass, [3] flat carry,
-    // [4] block fix-up, [5] inverse column DFT + carry, [6] carry-in +
-    // forward column DFT ([5], [6] are 0 without the split outer pass).
-    // Slot 0's residue is saved and restored.
+    // events between consecutive kernels of `reps` real plain squarings on
+    // this group's stream (kernels in their steady-state order, each on its
+    // predecessor's output): [0] forward column pass, [1] row pass, [2]
+    // inverse column pass, [3] flat carry, [4] block fix-up, [5] inverse
+    // column DFT + carry, [6] carry-in + forward column DFT ([5], [6] are 0
+    // without the split outer pass; with it, [0], [2]-[4] come from the one
+    // entry/exit pass around the loop).  Slot 0's residue is saved and
+    // restored; other slots are squared `reps` times.
     void kernel_times(int reps, double* ms);
     static constexpr int kNumKernels = 7;
     static constexpr int kChunk = 64;
@@ -138,5 +141,9 @@ class Group {
 };
 
 void check(hipError_t e, const char* what);
+
+// Average ms of a plain device-to-device copy of `bytes` (reads and writes
+// that many bytes) over `reps` launches after one warm-up launch.
+double copy_probe(int device, size_t bytes, int reps);
 
 }  // namespace pleng

@@ -602,6 +602,14 @@ __global__ void k_fill_u64(unsigned long long* a, int count)
     if (i < count) a[i] = 0ull;
 }
 
+// Plain streaming copy of n 16-byte elements (grid-stride): reads and writes
+// exactly 16*n bytes, the reference point for bandwidth and byte counters.
+__global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ a, double2* __restrict__ b, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
 // End of a scheduled step: a slot that has carried total-1 squarings stops
 // (its last squaring is carried by the leave sequence).
 __global__ void k_advance(SlotState* st, const SlotDesc* desc, unsigned slots)
@@ -1232,29 +1240,61 @@ void Group::kernel_times(int reps, double* ms)
     int64_t saved_xhi = 0;
     download(0, saved, saved_xhi);
     const Params p = params(0, false, 0);
-    hipEvent_t a, b;
-    check(hipEventCreate(&a), "hipEventCreate");
-    check(hipEventCreate(&b), "hipEventCreate");
-    for (int kind = 0; kind < kNumKernels; ++kind) {
-        if (kind >= kInvCarry && !plan_.nt) {
-            ms[kind] = 0.0;
-            continue;
+    // Real squarings in their steady-state order, an event between every two
+    // kernels: each kernel is timed on the data its predecessor just wrote,
+    // as it runs inside a squaring chunk.  The first and last kernels of a
+    // chunk (forward column pass from digits, inverse column + carry + fix-up
+    // back to digits) are timed on the same pass around the loop.
+    std::vector<int> seq;
+    if (plan_.nt)
+        seq = { kInvCarry, kAdjFwd, kRow };
+    else
+        seq = { kInvCol, kCarry, kFixup, kFwdCol, kRow };
+    const int m = (int)seq.size();
+    std::vector<hipEvent_t> ev((size_t)(m + 1) * (size_t)reps + 8);
+    for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+    for (int kind = 0; kind < kNumKernels; ++kind) ms[kind] = 0.0;
+    // enter: the first squaring's forward transform (timed once, then the
+    // forward column pass as the unfused steady state times it)
+    check(hipEventRecord(ev[0], stream_), "hipEventRecord");
+    launch(kFwdCol, p, cap_);
+    check(hipEventRecord(ev[1], stream_), "hipEventRecord");
+    launch(kRow, p, cap_);
+    size_t e = 2;
+    check(hipEventRecord(ev[e], stream_), "hipEventRecord");
+    const size_t loop0 = e;
+    for (int r = 0; r < reps; ++r)
+        for (int j = 0; j < m; ++j) {
+            launch(seq[(size_t)j], p, cap_);
+            check(hipEventRecord(ev[++e], stream_), "hipEventRecord");
         }
-        for (int pass = 0; pass < 2; ++pass) {
-            const int cnt = pass == 0 ? 1 : reps;
-            if (pass == 1) check(hipEventRecord(a, stream_), "hipEventRecord");
-            for (int i = 0; i < cnt; ++i) launch(kind, p, cap_);
-            if (pass == 1) check(hipEventRecord(b, stream_), "hipEventRecord");
-        }
-        check(hipEventSynchronize(b), "hipEventSynchronize");
+    // leave: back to digits
+    const size_t tail0 = e;
+    launch(kInvCol, p, cap_);
+    check(hipEventRecord(ev[++e], stream_), "hipEventRecord");
+    launch(kCarry, p, cap_);
+    check(hipEventRecord(ev[++e], stream_), "hipEventRecord");
+    launch(kFixup, p, cap_);
+    check(hipEventRecord(ev[++e], stream_), "hipEventRecord");
+    check(hipEventSynchronize(ev[e]), "hipEventSynchronize");
+    auto span = [&](size_t i, size_t j) {
         float t = 0;
-        check(hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
-        ms[kind] = (double)t / reps;
+        check(hipEventElapsedTime(&t, ev[i], ev[j]), "hipEventElapsedTime");
+        return (double)t;
+    };
+    for (int r = 0; r < reps; ++r)
+        for (int j = 0; j < m; ++j) {
+            const size_t i = loop0 + (size_t)r * (size_t)m + (size_t)j;
+            ms[seq[(size_t)j]] += span(i, i + 1) / reps;
+        }
+    if (plan_.nt) {
+        ms[kFwdCol] = span(0, 1);
+        ms[kInvCol] = span(tail0, tail0 + 1);
+        ms[kCarry] = span(tail0 + 1, tail0 + 2);
+        ms[kFixup] = span(tail0 + 2, tail0 + 3);
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    // the timing launches leave arbitrary data and status behind
-    clear_status();
+    for (auto& ee : ev) (void)hipEventDestroy(ee);
+    if (status()) clear_status();
     upload(0, saved, saved_xhi);
 }
 
@@ -1269,6 +1309,47 @@ void Group::clear_status()
 {
     check(hipMemsetAsync(status_, 0, sizeof(int), stream_), "status reset");
     check(hipStreamSynchronize(stream_), "status sync");
+}
+
+double copy_probe(int device, size_t bytes, int reps)
+{
+    check(hipSetDevice(device), "hipSetDevice");
+    const size_t n = bytes / sizeof(double2);
+    if (n == 0 || reps <= 0) throw std::runtime_error("copy_probe: empty copy");
+    double2 *a = nullptr, *b = nullptr;
+    check(hipMalloc(&a, n * sizeof(double2)), "hipMalloc");
+    hipError_t e = hipMalloc(&b, n * sizeof(double2));
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        check(e, "hipMalloc");
+    }
+    hipStream_t st = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    float ms = 0;
+    try {
+        check(hipStreamCreate(&st), "hipStreamCreate");
+        check(hipEventCreate(&t0), "hipEventCreate");
+        check(hipEventCreate(&t1), "hipEventCreate");
+        check(hipMemsetAsync(a, 0, n * sizeof(double2), st), "hipMemsetAsync");
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, st, a, b, n);
+        check(hipEventRecord(t0, st), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, st, a, b, n);
+        check(hipEventRecord(t1, st), "hipEventRecord");
+        check(hipEventSynchronize(t1), "hipEventSynchronize");
+        check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
+    } catch (...) {
+        if (st) (void)hipStreamSynchronize(st);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        throw;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return (double)ms / reps;
 }
 
 }  // namespace pleng
