@@ -115,6 +115,8 @@ class Group {
     void enqueue_step_chunk(int g);
     void build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g);
     double read_err(int g, double* trace, double* slot_err);
+    void enqueue_err_clear();
+    void enqueue_err_reduce(uint32_t slot0, uint32_t slots);
 
     Plan plan_;
     uint32_t cap_;
@@ -127,7 +129,9 @@ class Group {
     long long* runc_ = nullptr;
     int* rdig_ = nullptr;
     long long* xhi_ = nullptr;
-    unsigned long long* err_ = nullptr;  // cap x kChunk (bit patterns of non-negative doubles)
+    unsigned long long* err_ = nullptr;   // cap x kChunk (bit patterns of non-negative doubles)
+    unsigned long long* errp_ = nullptr;  // cap x eb_ x kChunk per-block partials
+    uint32_t eb_ = 1;                     // blocks that report round-off per squaring
     int* status_ = nullptr;
     int32_t* digits_dev_ = nullptr;
     SlotDesc* desc_ = nullptr;

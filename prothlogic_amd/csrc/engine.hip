@@ -71,11 +71,20 @@ struct Params {
     const double* ipw_hi;   // 2^(-j*2^s/L) / L
     const double2* om_lo;   // exp(i*pi*j/(2L))
     const double2* om_hi;   // exp(i*pi*j*2^s/(2L))
+    // Factored weights of digit m = r*C + c (per slot, they depend on n):
+    // fwd a_m*w^m = wrf[r]*wcf[c] * (1/2 when F_r + G_c >= L), inverse
+    // conj(w^m)/(L*a_m) = wri[r]*wci[c] * (2 when F_r + G_c >= L), with
+    // F_r = (-n*r*C) mod L, G_c = (-n*c) mod L.
+    const double2* wrf;
+    const double2* wri;
+    const double2* wcf;
+    const double2* wci;
+    const double2* tcq;     // W_L^(c*q*R/16), c < C, q < 16 (index c*16 + q)
     long long* carries;     // flat carry kernel: 3 per block (own carry-outs of its last run)
     long long* runc;        // split outer pass: 3 carry-outs per run (block-major, then row)
     int* dig;               // split outer pass: NT int32 digits per run (same order)
     long long* xhi;
-    unsigned long long* err;
+    unsigned long long* err;  // per-block round-off partials: block b of this squaring at err[b*kChunk]
     int* status;
     const SlotDesc* desc;
     SlotState* state;
@@ -96,6 +105,8 @@ struct DeviceTables {
     double2* tw;
     double2 *twl_lo, *twl_hi, *om_lo, *om_hi;
     double *pw_lo, *pw_hi, *ipw_lo, *ipw_hi;
+    double2* tcq;
+    double2 *wrf, *wri, *wcf, *wci;  // per slot: R, R, C, C entries
 };
 
 // The slot's view of the launch parameters.
@@ -108,7 +119,11 @@ __device__ __forceinline__ Params bind(const Params& base, unsigned y)
     p.runc += (size_t)slot * base.runc_stride;
     p.carries += (size_t)slot * base.carr_stride;
     p.xhi += slot;
-    p.err += (size_t)slot * base.err_stride;
+    p.err += (size_t)slot * base.err_stride;  // err_stride = blocks * kChunk
+    p.wrf += (size_t)slot * base.R;
+    p.wri += (size_t)slot * base.R;
+    p.wcf += (size_t)slot * base.C;
+    p.wci += (size_t)slot * base.C;
     const SlotDesc d = base.desc[slot];
     p.n = d.n;
     p.k = d.k;
@@ -229,6 +244,18 @@ __device__ __forceinline__ __int128 bal128(__int128 v, int b, __int128& carry)
     return v - q * ((__int128)1 << b);
 }
 
+__device__ __forceinline__ long long bal64(long long v, int b, long long& carry)
+{
+    const long long q = (v + (1ll << (b - 1))) >> b;
+    carry = q;
+    return v - q * (1ll << b);
+}
+
+__device__ __forceinline__ bool fits62(__int128 v)
+{
+    return v < ((__int128)1 << 62) && v > -((__int128)1 << 62);
+}
+
 template <int W>
 __device__ __forceinline__ void ripple(const Params& p, unsigned m0, __int128 c2in, __int128 extra, long long (&nu)[W])
 {
@@ -240,6 +267,28 @@ __device__ __forceinline__ void ripple(const Params& p, unsigned m0, __int128 c2
         walk.next();
     }
     const long long km = p.k * p.mult;
+    if (fits62(c2in) && fits62(extra)) {
+        // 64-bit path: |c2in|, |extra| < 2^62 and |km*d| < 2^(32+7+18) keep
+        // every sum below 2^63.
+        long long rem = (long long)c2in;
+        long long a[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            long long c;
+            const long long d = bal64(rem, bw[j], c);
+            a[j] = -km * d;
+            rem = c;
+        }
+        long long c = (long long)extra;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            long long cc;
+            nu[j] = bal64(a[j] + c, bw[j], cc);
+            c = cc;
+        }
+        if (rem != 0 || c != 0) atomicOr(p.status, 1);
+        return;
+    }
     __int128 rem = c2in;
     long long a[W];
 #pragma unroll
@@ -283,11 +332,13 @@ __global__ __launch_bounds__(TPB) void k_fwd_col(Params pb)
     const Params p = bind(pb, blockIdx.y);
     if (!p.active) return;
     __shared__ double2 lds[LdsLayout<R, NT>::kSlots];
+    StageTw<R, NT, TPB> t;
+    load_stage_tw(t, p.tw);
     const unsigned c0 = blockIdx.x * NT;
     const double2* __restrict__ z = p.z;
     dft<R, NT, TPB, false>(
-        lds, p.tw, [&](int tr, int idx) { return z[(size_t)idx * p.C + c0 + tr]; },
-        [&](int tr, int idx, double2 v) {
+        lds, t, [&](int tr, int idx, int, int) { return z[(size_t)idx * p.C + c0 + tr]; },
+        [&](int tr, int idx, double2 v, int, int) {
             const unsigned c = c0 + tr;
             p.z[(size_t)idx * p.C + c] = cmul(v, twL(p, c * (unsigned)idx));
         });
@@ -299,19 +350,22 @@ __global__ __launch_bounds__(TPB) void k_inv_col(Params pb)
     const Params p = bind(pb, blockIdx.y);
     if (!p.active) return;
     __shared__ double2 lds[LdsLayout<R, NT>::kSlots];
+    StageTw<R, NT, TPB> t;
+    load_stage_tw(t, p.tw);
     const unsigned c0 = blockIdx.x * NT;
     const double2* __restrict__ z = p.z;
     dft<R, NT, TPB, true>(
-        lds, p.tw,
-        [&](int tr, int idx) {
+        lds, t,
+        [&](int tr, int idx, int, int) {
             const unsigned c = c0 + tr;
             return cmulc(z[(size_t)idx * p.C + c], twL(p, c * (unsigned)idx));
         },
-        [&](int tr, int idx, double2 v) { p.z[(size_t)idx * p.C + c0 + tr] = v; });
+        [&](int tr, int idx, double2 v, int, int) { p.z[(size_t)idx * p.C + c0 + tr] = v; });
 }
 
 // Row pass: NT rows; forward C-point DFT, Z <- Z*(Z + 2i*X_hi) folded into
-// the last forward stage, inverse C-point DFT.
+// the last forward stage, inverse C-point DFT.  The stage twiddles are read
+// once and serve both directions.
 template <int C, int NT, int TPB>
 __global__ __launch_bounds__(TPB) void k_row(Params pb)
 {
@@ -319,17 +373,39 @@ __global__ __launch_bounds__(TPB) void k_row(Params pb)
     if (!p.active) return;
     using Lay = LdsLayout<C, NT>;
     __shared__ double2 lds[Lay::kSlots];
+    StageTw<C, NT, TPB, PL_ROW_RADIX> t;
+    load_stage_tw(t, p.tw);
     const size_t base = (size_t)blockIdx.x * NT * C;
     const double2* __restrict__ z = p.z;
     const double x2 = 2.0 * (double)(*p.xhi);
     dft<C, NT, TPB, false, false, true>(
-        lds, p.tw, [&](int tr, int idx) { return z[base + (size_t)tr * C + idx]; },
-        [&](int tr, int idx, double2 X) { lds[Lay::at(tr, idx)] = cmul(X, make_double2(X.x, X.y + x2)); });
+        lds, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
+        [&](int tr, int idx, double2 X, int, int) { lds[Lay::at(tr, idx)] = cmul(X, make_double2(X.x, X.y + x2)); });
     __syncthreads();
     dft<C, NT, TPB, true, true>(
-        lds, p.tw, [&](int tr, int idx) { return lds[Lay::at(tr, idx)]; },
-        [&](int tr, int idx, double2 v) { p.z[base + (size_t)tr * C + idx] = v; });
+        lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; },
+        [&](int tr, int idx, double2 v, int, int) { p.z[base + (size_t)tr * C + idx] = v; });
 }
+
+// Four-step twiddles W_L^(c*idx) of the points one thread moves in the
+// first (or last) stage of a column DFT, where idx = j + r*R/P:
+// W_L^(c*j) (two-level table) times W_L^(c*r*R/P) = tcq[c*16 + r*16/P].
+template <int R, int NT, int TPB, int P>
+struct ColTw {
+    using S = StageShape<R, NT, TPB, P>;
+    double2 w[S::PER][P];
+    __device__ __forceinline__ void load(const Params& p, unsigned c)
+    {
+#pragma unroll
+        for (int u = 0; u < S::PER; ++u) {
+            if (!S::live(u)) continue;
+            const double2 a = twL(p, c * (unsigned)S::j(u));
+            w[u][0] = a;
+#pragma unroll
+            for (int r = 1; r < P; ++r) w[u][r] = cmul(a, p.tcq[(size_t)c * 16 + r * (16 / P)]);
+        }
+    }
+};
 
 // ------------------------------------------------------------- flat carry
 
@@ -404,7 +480,7 @@ __global__ __launch_bounds__(TPB) void k_carry(Params pb)
         double e = s_err[0];
 #pragma unroll
         for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
-        atomicMax(p.err, (unsigned long long)__double_as_longlong(e));
+        p.err[(size_t)blockIdx.x * Group::kChunk] = (unsigned long long)__double_as_longlong(e);
     }
 }
 
@@ -413,7 +489,7 @@ __global__ __launch_bounds__(TPB) void k_carry(Params pb)
 // part of T_hi) and c2_top + X_hi^2 (the second fold's top scalar); the new
 // X_hi is the top run's third carry.
 template <int RUN, int CTPB>
-__global__ void k_fixup(Params pb, unsigned nblocks)
+__global__ __launch_bounds__(64) void k_fixup(Params pb, unsigned nblocks)
 {
     const Params p = bind(pb, blockIdx.y);
     if (!p.active) return;
@@ -452,7 +528,9 @@ __global__ void k_fixup(Params pb, unsigned nblocks)
 // digits at dig + (b*R + r)*NT and its three carry-outs at runc + (b*R + r)*3.
 
 // Inverse column DFT, unweight, round (max round-off), own carries of every
-// run; writes the runs' digits (before their carry-in) and carry-outs.
+// run; writes the runs' digits (before their carry-in) and carry-outs.  All
+// table values the block needs (stage twiddles, four-step twiddles, row and
+// column weights) are requested before the transform, beside the data.
 template <int R, int NT, int TPB>
 __global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
 {
@@ -460,20 +538,35 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
     if (!p.active) return;
     using Lay = LdsLayout<R, NT>;
     constexpr int RPT = (R + TPB - 1) / TPB;
+    constexpr int P0 = Radix<R>::first();
     __shared__ double2 lds[Lay::kSlots];
     __shared__ double s_err[TPB / 64];
     const unsigned b = blockIdx.x;
     const unsigned c0 = b * NT;
     const double2* __restrict__ z = p.z;
+    StageTw<R, NT, TPB> t;
+    load_stage_tw(t, p.tw);
+    ColTw<R, NT, TPB, P0> ct;
+    ct.load(p, c0 + threadIdx.x % NT);
+    double2 wr[RPT];
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        wr[u] = r < (unsigned)R ? p.wri[r] : make_double2(0.0, 0.0);
+    }
+    double2 wc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) wc[j] = p.wci[c0 + j];
     dft<R, NT, TPB, true, false, true>(
-        lds, p.tw,
-        [&](int tr, int idx) {
-            const unsigned c = c0 + tr;
-            return cmulc(z[(size_t)idx * p.C + c], twL(p, c * (unsigned)idx));
-        },
-        [&](int tr, int idx, double2 v) { lds[Lay::at(tr, idx)] = v; });
+        lds, t, [&](int tr, int idx, int u, int r) { return cmulc(z[(size_t)idx * p.C + c0 + tr], ct.w[u][r]); },
+        [&](int tr, int idx, double2 v, int, int) { lds[Lay::at(tr, idx)] = v; });
     __syncthreads();
 
+    const unsigned long long Lm = p.L - 1;
+    const unsigned nr = (unsigned)(p.n & Lm);
+    unsigned G[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) G[j] = (unsigned)((0ull - p.n * (unsigned long long)(c0 + j)) & Lm);
     double emax = 0.0;
     const double kd = (double)p.k, kmd = (double)(p.k * p.mult);
     const double tb0 = ldexp(1.0, (int)(p.n >> p.logL)), tb1 = 2.0 * tb0;
@@ -482,18 +575,21 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
     for (int u = 0; u < RPT; ++u) {
         const unsigned r = threadIdx.x + u * TPB;
         if (r >= (unsigned)R) break;
-        const unsigned m0 = r * p.C + c0;
+        const unsigned F = (unsigned)((0ull - p.n * (unsigned long long)r * p.C) & Lm);
         double q1 = 0, q2 = 0, q3 = 0;
-        DigitWalk walk(p, m0);
         int dg[NT];
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
-            const double2 v = cmul(lds[Lay::at(j, r)], inv_weight_f(p, m0 + j, walk.f));
+            unsigned f = F + G[j];
+            const bool wrap = f >= p.L;
+            if (wrap) f -= p.L;
+            double2 w = cmul(wr[u], wc[j]);
+            if (wrap) w = make_double2(2.0 * w.x, 2.0 * w.y);
+            const double2 v = cmul(lds[Lay::at(j, r)], w);
             const double tr = rint(v.x), hr = rint(v.y);
             emax = fmax(emax, fmax(fabs(v.x - tr), fabs(v.y - hr)));
-            const bool wd = walk.wide();
+            const bool wd = f < nr;
             dg[j] = (int)carry3f(tr, hr, kd, kmd, wd ? tb1 : tb0, wd ? itb1 : itb0, q1, q2, q3);
-            walk.next();
         }
         const size_t run = (size_t)b * R + r;
         int4* dst = reinterpret_cast<int4*>(p.dig + run * NT);
@@ -511,7 +607,7 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
         double e = s_err[0];
 #pragma unroll
         for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
-        atomicMax(p.err, (unsigned long long)__double_as_longlong(e));
+        p.err[(size_t)blockIdx.x * Group::kChunk] = (unsigned long long)__double_as_longlong(e);
     }
 }
 
@@ -526,17 +622,30 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
     if (!p.active) return;
     using Lay = LdsLayout<R, NT>;
     constexpr int RPT = (R + TPB - 1) / TPB;
+    constexpr int PL = Radix<R>::last();
     __shared__ double2 lds[Lay::kSlots];
     const unsigned nb = p.C / NT;
     const unsigned b = blockIdx.x;
     const unsigned c0 = b * NT;
     const long long k = p.k;
     const long long* top = p.runc + 3 * ((size_t)(nb - 1) * R + (R - 1));
+    StageTw<R, NT, TPB> t;
+    load_stage_tw(t, p.tw);
+    ColTw<R, NT, TPB, PL> ct;
+    ct.load(p, c0 + threadIdx.x % NT);
+    double2 wc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) wc[j] = p.wcf[c0 + j];
+    const unsigned long long Lm = p.L - 1;
+    unsigned G[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) G[j] = (unsigned)((0ull - p.n * (unsigned long long)(c0 + j)) & Lm);
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
         const unsigned r = threadIdx.x + u * TPB;
         if (r >= (unsigned)R) break;
         const unsigned m0 = r * p.C + c0;
+        const double2 wr = p.wrf[r];
         __int128 c2in, extra;
         if (b == 0 && r == 0) {
             top_rule(p, top, c2in, extra);
@@ -557,11 +666,11 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
             dg[4 * j + 2] = q.z;
             dg[4 * j + 3] = q.w;
         }
-        DigitWalk walk(p, m0);
+        const unsigned F = (unsigned)((0ull - p.n * (unsigned long long)r * p.C) & Lm);
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
-            const double2 w = fwd_weight_f(p, m0 + j, walk.f);
-            walk.next();
+            double2 w = cmul(wr, wc[j]);
+            if (F + G[j] >= p.L) w = make_double2(0.5 * w.x, 0.5 * w.y);
             const double xd = (double)dg[j] + (double)nu[j];
             lds[Lay::at(j, r)] = make_double2(xd * w.x, xd * w.y);
         }
@@ -570,10 +679,9 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
     if (b == 0 && threadIdx.x == 0) *p.xhi = top[2];
     __syncthreads();
     dft<R, NT, TPB, false, true, false>(
-        lds, p.tw, [&](int tr, int idx) { return lds[Lay::at(tr, idx)]; },
-        [&](int tr, int idx, double2 v) {
-            const unsigned c = c0 + tr;
-            p.z[(size_t)idx * p.C + c] = cmul(v, twL(p, c * (unsigned)idx));
+        lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; },
+        [&](int tr, int idx, double2 v, int u, int r) {
+            p.z[(size_t)idx * p.C + c0 + tr] = cmul(v, ct.w[u][r]);
         });
 }
 
@@ -602,6 +710,39 @@ __global__ void k_fill_u64(unsigned long long* a, int count)
     if (i < count) a[i] = 0ull;
 }
 
+// Round-off of every squaring of a chunk: column i of slot s is the max of
+// the blocks' partials (non-negative doubles compare as their bit patterns).
+// One block of 1024 threads per slot: thread t takes column t % 64 of the
+// blocks t/64, t/64 + 16, ... (partials [slot][block][column], so a wave
+// reads 512 contiguous bytes), then the 16 partial maxima of each column
+// meet in LDS.
+__global__ __launch_bounds__(1024) void k_err_reduce(const unsigned long long* __restrict__ part,
+                                                     unsigned long long* __restrict__ out, unsigned blocks,
+                                                     unsigned slot0)
+{
+    constexpr int K = Group::kChunk, GROUPS = 1024 / K;
+    __shared__ unsigned long long red[GROUPS][K];
+    const unsigned s = slot0 + blockIdx.x, i = threadIdx.x % K, g0 = threadIdx.x / K;
+    const unsigned long long* src = part + (size_t)s * blocks * K + i;
+    unsigned long long m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    unsigned b = g0;
+    for (; b + 3 * GROUPS < blocks; b += 4 * GROUPS) {
+        m0 = max(m0, src[(size_t)b * K]);
+        m1 = max(m1, src[(size_t)(b + GROUPS) * K]);
+        m2 = max(m2, src[(size_t)(b + 2 * GROUPS) * K]);
+        m3 = max(m3, src[(size_t)(b + 3 * GROUPS) * K]);
+    }
+    for (; b < blocks; b += GROUPS) m0 = max(m0, src[(size_t)b * K]);
+    red[g0][i] = max(max(m0, m1), max(m2, m3));
+    __syncthreads();
+    if (g0 == 0) {
+        unsigned long long m = red[0][i];
+#pragma unroll
+        for (int g = 1; g < GROUPS; ++g) m = max(m, red[g][i]);
+        out[(size_t)s * K + i] = m;
+    }
+}
+
 // Plain streaming copy of n 16-byte elements (grid-stride): reads and writes
 // exactly 16*n bytes, the reference point for bandwidth and byte counters.
 __global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ a, double2* __restrict__ b, size_t n)
@@ -623,10 +764,11 @@ __global__ void k_advance(SlotState* st, const SlotDesc* desc, unsigned slots)
 
 // ------------------------------------------------------------- host side
 
-// One largest-radix butterfly per thread per stage, within [64, 1024].
-constexpr int tpb_for(int points)
+// One largest-radix butterfly per thread per stage, within [64, 512] (at
+// 1024 threads the 128-register limit would spill the register twiddles).
+constexpr int tpb_for(int points, int radix = PL_MAX_RADIX)
 {
-    return points / PL_MAX_RADIX < 64 ? 64 : (points / PL_MAX_RADIX > 1024 ? 1024 : points / PL_MAX_RADIX);
+    return points / radix < 64 ? 64 : (points / radix > 512 ? 512 : points / radix);
 }
 
 double max_bits_per_digit(uint32_t logL)
@@ -738,7 +880,7 @@ struct SplitCfg {
 template <int C>
 struct RowCfg {
     static constexpr int NT = (1024 / C) > 1 ? (1024 / C) : 1;
-    static constexpr int TPB = tpb_for(NT * C);
+    static constexpr int TPB = tpb_for(NT * C, PL_ROW_RADIX);
 };
 
 enum Kind { kFwdCol = 0, kRow = 1, kInvCol = 2, kCarry = 3, kFixup = 4, kInvCarry = 5, kAdjFwd = 6, kAdvance = 7 };
@@ -856,15 +998,51 @@ static void host_tables(const Plan& pl, std::vector<double2>& tw, std::vector<do
     }
 }
 
+// W_L^(c*q*R/16) = exp(-2*pi*i*c*q/(16*C)) for c < C, q < 16.
+static void host_tcq(const Plan& pl, std::vector<double2>& tcq)
+{
+    const long double PI = 3.141592653589793238462643383279502884L;
+    tcq.resize((size_t)pl.C * 16);
+    for (uint32_t c = 0; c < pl.C; ++c)
+        for (uint32_t q = 0; q < 16; ++q) {
+            const long double a = -2.0L * PI * (long double)((uint64_t)c * q) / (16.0L * (long double)pl.C);
+            tcq[(size_t)c * 16 + q] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+}
+
+// Factored weights of one slot (see Params): row r and column c factors of
+// a_m*w^m and conj(w^m)/(L*a_m) for digit m = r*C + c.
+static void host_slot_weights(const Plan& pl, uint64_t n, double2* wrf, double2* wri, double2* wcf, double2* wci)
+{
+    const long double PI = 3.141592653589793238462643383279502884L;
+    const long double L = (long double)pl.L;
+    const uint64_t Lm = pl.L - 1;
+    for (uint32_t r = 0; r < pl.R; ++r) {
+        const uint64_t F = (0ull - n * (uint64_t)r * pl.C) & Lm;
+        const long double a = exp2l((long double)F / L);
+        const long double o = PI * (long double)((uint64_t)r * pl.C) / (2.0L * L);
+        wrf[r] = make_double2((double)(a * cosl(o)), (double)(a * sinl(o)));
+        wri[r] = make_double2((double)(cosl(o) / a), (double)(-sinl(o) / a));
+    }
+    for (uint32_t c = 0; c < pl.C; ++c) {
+        const uint64_t G = (0ull - n * (uint64_t)c) & Lm;
+        const long double a = exp2l((long double)G / L);
+        const long double o = PI * (long double)c / (2.0L * L);
+        wcf[c] = make_double2((double)(a * cosl(o)), (double)(a * sinl(o)));
+        wci[c] = make_double2((double)(cosl(o) / (a * L)), (double)(-sinl(o) / (a * L)));
+    }
+}
+
 static uint32_t carry_tpb_for(const Plan& p) { return p.L >= p.run * 256u * 16u ? 256 : 64; }
 
 size_t Group::slot_bytes(const Plan& p)
 {
     // residue (16 B/pt) + split-pass digits (4 B/pt) + run carries (24 B per
-    // run) + flat-carry block carries + scalars
+    // run) + flat-carry block carries + factored weights + scalars
     const size_t runs = p.nt ? p.L / p.nt : 0;
     const uint32_t tpb = carry_tpb_for(p);
-    return (size_t)p.L * 16 + (size_t)p.L * 4 + runs * 24 + (size_t)(p.L / (p.run * tpb)) * 24 + 64 + kChunk * 8;
+    return (size_t)p.L * 16 + (size_t)p.L * 4 + runs * 24 + (size_t)(p.L / (p.run * tpb)) * 24 +
+           (size_t)(p.R + p.C) * 32 + 64 + kChunk * 8;
 }
 
 Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capacity), device_(device)
@@ -886,7 +1064,9 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     const size_t o_runc = off; off += al(sizeof(long long) * 3 * runs * cap_);
     const size_t o_rdig = off; off += al(p.nt ? sizeof(int) * (size_t)p.L * cap_ : 0);
     const size_t o_xhi = off; off += al(sizeof(long long) * cap_);
+    eb_ = std::max<uint32_t>(carry_blocks_, p.nt ? p.C / p.nt : 1);
     const size_t o_err = off; off += al(sizeof(unsigned long long) * kChunk * cap_);
+    const size_t o_errp = off; off += al(sizeof(unsigned long long) * kChunk * eb_ * cap_);
     const size_t o_st = off; off += al(sizeof(int));
     const size_t o_dig = off; off += al(sizeof(int32_t) * p.L);
     const size_t o_desc = off; off += al(sizeof(SlotDesc) * cap_);
@@ -901,6 +1081,13 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     const size_t o_phi = off; off += al(sizeof(double) * pw_hi.size());
     const size_t o_ilo = off; off += al(sizeof(double) * ipw_lo.size());
     const size_t o_ihi = off; off += al(sizeof(double) * ipw_hi.size());
+    std::vector<double2> tcq;
+    host_tcq(p, tcq);
+    const size_t o_tcq = off; off += al(sizeof(double2) * tcq.size());
+    const size_t o_wrf = off; off += al(sizeof(double2) * p.R * cap_);
+    const size_t o_wri = off; off += al(sizeof(double2) * p.R * cap_);
+    const size_t o_wcf = off; off += al(sizeof(double2) * p.C * cap_);
+    const size_t o_wci = off; off += al(sizeof(double2) * p.C * cap_);
     bytes_ = off;
     check(hipMalloc(&mem_, bytes_), "hipMalloc");
     char* b = static_cast<char*>(mem_);
@@ -910,6 +1097,7 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     rdig_ = reinterpret_cast<int*>(b + o_rdig);
     xhi_ = reinterpret_cast<long long*>(b + o_xhi);
     err_ = reinterpret_cast<unsigned long long*>(b + o_err);
+    errp_ = reinterpret_cast<unsigned long long*>(b + o_errp);
     status_ = reinterpret_cast<int*>(b + o_st);
     digits_dev_ = reinterpret_cast<int32_t*>(b + o_dig);
     desc_ = reinterpret_cast<SlotDesc*>(b + o_desc);
@@ -924,6 +1112,11 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     tabs_->pw_hi = reinterpret_cast<double*>(b + o_phi);
     tabs_->ipw_lo = reinterpret_cast<double*>(b + o_ilo);
     tabs_->ipw_hi = reinterpret_cast<double*>(b + o_ihi);
+    tabs_->tcq = reinterpret_cast<double2*>(b + o_tcq);
+    tabs_->wrf = reinterpret_cast<double2*>(b + o_wrf);
+    tabs_->wri = reinterpret_cast<double2*>(b + o_wri);
+    tabs_->wcf = reinterpret_cast<double2*>(b + o_wcf);
+    tabs_->wci = reinterpret_cast<double2*>(b + o_wci);
     auto up = [&](void* dst, const void* src, size_t n) {
         check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync H2D");
     };
@@ -936,6 +1129,23 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     up(tabs_->pw_hi, pw_hi.data(), sizeof(double) * pw_hi.size());
     up(tabs_->ipw_lo, ipw_lo.data(), sizeof(double) * ipw_lo.size());
     up(tabs_->ipw_hi, ipw_hi.data(), sizeof(double) * ipw_hi.size());
+    up(tabs_->tcq, tcq.data(), sizeof(double2) * tcq.size());
+    {
+        std::vector<double2> wrf((size_t)p.R * cap_), wri((size_t)p.R * cap_), wcf((size_t)p.C * cap_),
+            wci((size_t)p.C * cap_);
+        host_slot_weights(p, p.n, wrf.data(), wri.data(), wcf.data(), wci.data());
+        for (uint32_t i = 1; i < cap_; ++i) {
+            std::copy(wrf.begin(), wrf.begin() + p.R, wrf.begin() + (size_t)i * p.R);
+            std::copy(wri.begin(), wri.begin() + p.R, wri.begin() + (size_t)i * p.R);
+            std::copy(wcf.begin(), wcf.begin() + p.C, wcf.begin() + (size_t)i * p.C);
+            std::copy(wci.begin(), wci.begin() + p.C, wci.begin() + (size_t)i * p.C);
+        }
+        up(tabs_->wrf, wrf.data(), sizeof(double2) * wrf.size());
+        up(tabs_->wri, wri.data(), sizeof(double2) * wri.size());
+        up(tabs_->wcf, wcf.data(), sizeof(double2) * wcf.size());
+        up(tabs_->wci, wci.data(), sizeof(double2) * wci.size());
+        check(hipStreamSynchronize(stream_), "table upload");
+    }
     check(hipMemsetAsync(mem_, 0, zero_end, stream_), "hipMemsetAsync");
     check(hipStreamSynchronize(stream_), "init sync");
     desc_host_.assign(cap_, SlotDesc{ p.n, (long long)p.k, 0, 0, 0 });
@@ -967,17 +1177,22 @@ Params Group::params(uint32_t slot0, bool check_active, uint32_t err_col) const
     p.ipw_hi = tabs_->ipw_hi;
     p.om_lo = tabs_->om_lo;
     p.om_hi = tabs_->om_hi;
+    p.tcq = tabs_->tcq;
+    p.wrf = tabs_->wrf;
+    p.wri = tabs_->wri;
+    p.wcf = tabs_->wcf;
+    p.wci = tabs_->wci;
     p.carries = carries_;
     p.runc = runc_;
     p.dig = rdig_;
     p.xhi = xhi_;
-    p.err = err_ + err_col;
+    p.err = errp_ + err_col;
     p.status = status_;
     p.desc = desc_;
     p.state = state_;
     p.carr_stride = 3 * (size_t)carry_blocks_;
     p.runc_stride = plan_.nt ? 3 * (size_t)(plan_.L / plan_.nt) : 0;
-    p.err_stride = kChunk;
+    p.err_stride = kChunk * eb_;
     p.slot0 = slot0;
     p.sched = check_active ? 1 : 0;
     p.n = plan_.n;
@@ -1025,6 +1240,16 @@ void Group::set_slot(uint32_t slot, uint32_t n, uint64_t k, uint32_t a, uint64_t
     desc_host_[slot] = SlotDesc{ n, (long long)k, a, top, total };
     check(hipMemcpyAsync(desc_ + slot, &desc_host_[slot], sizeof(SlotDesc), hipMemcpyHostToDevice, stream_),
           "desc upload");
+    std::vector<double2> wrf(plan_.R), wri(plan_.R), wcf(plan_.C), wci(plan_.C);
+    host_slot_weights(plan_, n, wrf.data(), wri.data(), wcf.data(), wci.data());
+    auto up = [&](double2* dst, const std::vector<double2>& v) {
+        check(hipMemcpyAsync(dst, v.data(), sizeof(double2) * v.size(), hipMemcpyHostToDevice, stream_),
+              "weight upload");
+    };
+    up(tabs_->wrf + (size_t)slot * plan_.R, wrf);
+    up(tabs_->wri + (size_t)slot * plan_.R, wri);
+    up(tabs_->wcf + (size_t)slot * plan_.C, wcf);
+    up(tabs_->wci + (size_t)slot * plan_.C, wci);
     check(hipStreamSynchronize(stream_), "desc sync");
 }
 
@@ -1111,12 +1336,25 @@ void Group::enqueue_step_chunk(int g)
 void Group::build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g)
 {
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    const int cnt = (int)(kChunk * cap_);
-    hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, err_, cnt);
+    enqueue_err_clear();
     if (stepping) enqueue_step_chunk(g);
     else enqueue_plain_chunk(g);
+    enqueue_err_reduce(0, cap_);
     check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
     check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+// Zero the round-off partials of every slot (a fill kernel, so the chunk
+// can be graph-captured).
+void Group::enqueue_err_clear()
+{
+    const int cnt = (int)((size_t)kChunk * eb_ * cap_);
+    hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, errp_, cnt);
+}
+
+void Group::enqueue_err_reduce(uint32_t slot0, uint32_t slots)
+{
+    hipLaunchKernelGGL(k_err_reduce, dim3(slots), dim3(1024), 0, stream_, errp_, err_, eb_, slot0);
 }
 
 // Reads the chunk's err columns: trace gets slot 0's per-column values,
@@ -1153,9 +1391,9 @@ double Group::square(uint64_t iters, double* err_trace)
             if (!plain_exec_) build_graph(plain_exec_, plain_graph_, false, kChunk);
             check(hipGraphLaunch(plain_exec_, stream_), "hipGraphLaunch");
         } else {
-            const int cnt = (int)(kChunk * cap_);
-            hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, err_, cnt);
+            enqueue_err_clear();
             enqueue_plain_chunk(g);
+            enqueue_err_reduce(0, cap_);
             check(hipGetLastError(), "square launch");
         }
         emax = std::max(emax, read_err(g, err_trace ? err_trace + done : nullptr, nullptr));
@@ -1185,9 +1423,9 @@ double Group::step(uint32_t steps, double* slot_err)
             if (!step_exec_) build_graph(step_exec_, step_graph_, true, kChunk);
             check(hipGraphLaunch(step_exec_, stream_), "hipGraphLaunch");
         } else {
-            const int cnt = (int)(kChunk * cap_);
-            hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, err_, cnt);
+            enqueue_err_clear();
             enqueue_step_chunk(g);
+            enqueue_err_reduce(0, cap_);
             check(hipGetLastError(), "step launch");
         }
         emax = std::max(emax, read_err(g, nullptr, tmp.data()));
@@ -1201,8 +1439,10 @@ double Group::step(uint32_t steps, double* slot_err)
 double Group::leave(uint32_t slot)
 {
     check(hipSetDevice(device_), "hipSetDevice");
-    unsigned long long zero = 0;
-    check(hipMemcpyAsync(err_ + (size_t)slot * kChunk, &zero, sizeof zero, hipMemcpyHostToDevice, stream_), "err reset");
+    {
+        const int cnt = (int)((size_t)kChunk * eb_);
+        hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, errp_ + (size_t)slot * cnt, cnt);
+    }
     // the schedule multiplier of the last squaring comes from the slot's state
     Params p = params(slot, true, 0);
     p.sched = 1;
@@ -1215,6 +1455,7 @@ double Group::leave(uint32_t slot)
     launch(kInvCol, p, 1);
     launch(kCarry, p, 1);
     launch(kFixup, p, 1);
+    enqueue_err_reduce(slot, 1);
     SlotState fin{ st.carried + 1, 0, 0 };
     check(hipMemcpyAsync(state_ + slot, &fin, sizeof fin, hipMemcpyHostToDevice, stream_), "state H2D");
     unsigned long long e = 0;

@@ -4,17 +4,29 @@
 // Thread t of a TPB-thread workgroup owns transform tr = t % NT and the
 // butterflies j = t/NT + u*(TPB/NT) of every stage, so a wavefront touches
 // NT adjacent transforms at each index: global accesses of a column tile are
-// NT*16 contiguous bytes, and LDS accesses hit consecutive 16-byte slots.
+// NT*16 contiguous bytes, and LDS accesses hit neighbouring 16-byte slots.
 //
-// LDS layout: element idx of transform tr lives at NT*(idx + idx/16) + tr.
-// The one-slot pad every 16 elements keeps the radix-16 write pattern
-// idx = 16*j + r conflict-free (lanes with successive j land NT slots apart).
+// LDS layout: element idx of transform tr is slot s = NT*idx + tr, XOR
+// swizzled within aligned groups of 2^PL_LDS_SWZ slots (s ^ ((s >> SWZ) &
+// (2^SWZ - 1))).  The swizzle keeps the three access patterns free of bank
+// conflicts: lanes stepping tr (every column tile), lanes stepping idx by the
+// stage radix (first-stage stores), and lanes stepping the transform index
+// while each reads NT consecutive slots (the row-per-thread carry phase).
 //
 // A transform is run as: first stage reading from a caller functor (global
 // memory or LDS), middle stages LDS -> LDS, last stage writing to a caller
-// functor (global memory or LDS).  Stage with span NS and radix P:
-// butterfly j reads j + r*N/P, twiddles by W_(NS*P)^(r*(j mod NS)), applies
-// the P-point DFT, writes (j/NS)*NS*P + (j mod NS) + r*NS (natural order out).
+// functor (global memory or LDS).  Functors get (tr, idx, u, r): u and r
+// are the (unrolled, compile-time) butterfly and leg numbers, so callers can
+// keep per-point values in registers indexed by them.  Stage with span NS
+// and radix P: butterfly j reads j + r*N/P, twiddles by W_(NS*P)^(r*(j mod
+// NS)), applies the P-point DFT, writes (j/NS)*NS*P + (j mod NS) + r*NS
+// (natural order out).  The first stage therefore reads idx = j + r*N/P0 and
+// the last writes idx = j + r*N/P_last.
+//
+// Stage twiddles are loaded once per kernel into registers (StageTw) before
+// the data arrive: per stage and butterfly the powers W^e, W^2e, W^4e, W^8e
+// come from the 4096-point table and the other powers are their products,
+// formed at use; the inverse transform uses the same registers conjugated.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -148,67 +160,153 @@ __device__ __forceinline__ void dftp(double2* v)
 #ifndef PL_MAX_RADIX
 #define PL_MAX_RADIX 8
 #endif
+#ifndef PL_LDS_SWZ
+#define PL_LDS_SWZ 3
+#endif
+#ifndef PL_ROW_RADIX
+#define PL_ROW_RADIX PL_MAX_RADIX
+#endif
 
-template <int N>
+template <int N, int MAXR = PL_MAX_RADIX>
 struct Radix {
-    // radix of the stage whose span is NS (largest power of two up to
-    // PL_MAX_RADIX that still fits)
+    // radix of the stage whose span is NS (largest power of two up to MAXR
+    // that still fits)
     static constexpr int of(int NS)
     {
-        return (N / NS) >= 16 && PL_MAX_RADIX >= 16 ? 16
-             : (N / NS) >= 8 && PL_MAX_RADIX >= 8   ? 8
-             : (N / NS) >= 4                        ? 4
-                                                    : 2;
+        return (N / NS) >= 16 && MAXR >= 16 ? 16
+             : (N / NS) >= 8 && MAXR >= 8   ? 8
+             : (N / NS) >= 4                ? 4
+                                            : 2;
+    }
+    static constexpr int first() { return of(1); }
+    static constexpr int last()
+    {
+        int ns = 1, p = of(1);
+        while (ns * p < N) {
+            ns *= p;
+            p = of(ns);
+        }
+        return p;
     }
 };
 
 template <int N, int NT>
 struct LdsLayout {
-    static constexpr int kSlots = NT * (N + N / 16);
-    __device__ static __forceinline__ int at(int tr, int idx) { return NT * (idx + (idx >> 4)) + tr; }
+    static constexpr int kSlots = NT * N;
+    static_assert(kSlots % (1 << PL_LDS_SWZ) == 0, "swizzle group must divide the buffer");
+    __device__ static __forceinline__ int at(int tr, int idx)
+    {
+        const int s = NT * idx + tr;
+        return s ^ ((s >> PL_LDS_SWZ) & ((1 << PL_LDS_SWZ) - 1));
+    }
 };
 
-// One stage.  ld(tr, idx) -> double2, st(tr, idx, v).  When both go to the
-// same LDS buffer the caller passes SYNC_BETWEEN = true so every read of the
-// stage completes before any write.
-template <int N, int NT, int TPB, int P, int NS, bool INV, bool SYNC_BETWEEN, class Ld, class St>
-__device__ __forceinline__ void stage(const double2* __restrict__ tw, Ld ld, St st)
-{
-    constexpr int G = TPB / NT;       // butterfly groups per transform
-    constexpr int NBT = N / P;        // butterflies per transform
-    constexpr int PER = NBT >= G ? NBT / G : 1;
-    constexpr int SCALE = 4096 / N;   // tw is the 4096-point table
+// Threads, butterflies per thread and the butterfly index of a stage.
+template <int N, int NT, int TPB, int P>
+struct StageShape {
+    static constexpr int G = TPB / NT;    // butterfly groups per transform
+    static constexpr int NBT = N / P;     // butterflies per transform
+    static constexpr int PER = NBT >= G ? NBT / G : 1;
     static_assert(TPB % NT == 0, "TPB must be a multiple of NT");
     static_assert(NBT < G || NBT % G == 0, "butterflies must split evenly");
+    __device__ static __forceinline__ int j(int u) { return (int)(threadIdx.x / NT) + u * G; }
+    __device__ static __forceinline__ bool live(int u) { return NBT >= G || j(u) < NBT; }
+};
+
+// Register-resident stage twiddles of one thread for every stage of an
+// N-point transform: for the stage with span NS and butterfly u, w[stage][u]
+// holds W^e, W^2e, W^4e, W^8e with W^e = W_(NS*P)^(j mod NS); the other
+// powers are formed where they are used (twiddle()).
+template <int N, int NT, int TPB, int MAXR = PL_MAX_RADIX>
+struct StageTw {
+    static constexpr int G = TPB / NT;
+    static constexpr int PMAX = (N / 2) >= G ? (N / 2) / G : 1;
+    double2 w[8][PMAX][4];
+};
+
+// All P-1 twiddles of a butterfly from its base powers b = {W^e, W^2e, W^4e, W^8e}.
+template <int P>
+__device__ __forceinline__ void twiddles(const double2 (&b)[4], double2* w)
+{
+    w[1] = b[0];
+    if constexpr (P >= 4) {
+        w[2] = b[1];
+        w[3] = cmul(b[0], b[1]);
+    }
+    if constexpr (P >= 8) {
+        w[4] = b[2];
+        w[5] = cmul(b[0], b[2]);
+        w[6] = cmul(b[1], b[2]);
+        w[7] = cmul(w[3], b[2]);
+    }
+    if constexpr (P >= 16) {
+        w[8] = b[3];
+#pragma unroll
+        for (int r = 9; r < 16; ++r) w[r] = cmul(w[r - 8], b[3]);
+    }
+}
+
+template <int N, int NT, int TPB, int MAXR, int NS, int SI>
+__device__ __forceinline__ void tw_fill(StageTw<N, NT, TPB, MAXR>& t, const double2* __restrict__ tw)
+{
+    constexpr int P = Radix<N, MAXR>::of(NS);
+    if constexpr (NS > 1) {
+        using S = StageShape<N, NT, TPB, P>;
+        constexpr int SCALE = 4096 / N;       // tw is the 4096-point table
+        constexpr int STEP = N / (NS * P);
+#pragma unroll
+        for (int u = 0; u < S::PER; ++u) {
+            if (!S::live(u)) continue;
+            // e < (N/P) * SCALE = 4096/P, so the largest index read,
+            // (P/2) * e, stays below 2048.
+            const int e = (S::j(u) % NS) * STEP * SCALE;
+            t.w[SI][u][0] = tw[e];
+            if constexpr (P >= 4) t.w[SI][u][1] = tw[2 * e];
+            if constexpr (P >= 8) t.w[SI][u][2] = tw[4 * e];
+            if constexpr (P >= 16) t.w[SI][u][3] = tw[8 * e];
+        }
+    }
+    if constexpr (NS * P < N) tw_fill<N, NT, TPB, MAXR, NS * P, SI + 1>(t, tw);
+}
+
+template <int N, int NT, int TPB, int MAXR>
+__device__ __forceinline__ void load_stage_tw(StageTw<N, NT, TPB, MAXR>& t, const double2* __restrict__ tw)
+{
+    tw_fill<N, NT, TPB, MAXR, 1, 0>(t, tw);
+}
+
+// One stage.  ld(tr, idx, u, r) -> double2, st(tr, idx, v, u, r).  When
+// both go to the same LDS buffer the caller passes SYNC_BETWEEN = true so
+// every read of the stage completes before any write.
+template <int N, int NT, int TPB, int MAXR, int P, int NS, int SI, bool INV, bool SYNC_BETWEEN, class Ld, class St>
+__device__ __forceinline__ void stage(const StageTw<N, NT, TPB, MAXR>& t, Ld ld, St st)
+{
+    using S = StageShape<N, NT, TPB, P>;
     const int tr = threadIdx.x % NT;
-    const int g = threadIdx.x / NT;
-    double2 v[PER][P];
+    double2 v[S::PER][P];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int j = g + u * G;
-        if (NBT >= G || j < NBT) {
+    for (int u = 0; u < S::PER; ++u) {
+        if (S::live(u)) {
+            const int j = S::j(u);
 #pragma unroll
-            for (int r = 0; r < P; ++r) v[u][r] = ld(tr, j + r * NBT);
+            for (int r = 0; r < P; ++r) v[u][r] = ld(tr, j + r * S::NBT, u, r);
             if constexpr (NS > 1) {
-                const int jj = j % NS;
+                double2 w[P];
+                twiddles<P>(t.w[SI][u], w);
 #pragma unroll
-                for (int r = 1; r < P; ++r) {
-                    double2 w = tw[(r * jj * (N / (NS * P))) * SCALE];
-                    if (INV) w.y = -w.y;
-                    v[u][r] = cmul(v[u][r], w);
-                }
+                for (int r = 1; r < P; ++r) v[u][r] = INV ? cmulc(v[u][r], w[r]) : cmul(v[u][r], w[r]);
             }
             dftp<P, INV>(v[u]);
         }
     }
     if constexpr (SYNC_BETWEEN) __syncthreads();
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int j = g + u * G;
-        if (NBT >= G || j < NBT) {
+    for (int u = 0; u < S::PER; ++u) {
+        if (S::live(u)) {
+            const int j = S::j(u);
             const int o = (j / NS) * NS * P + (j % NS);
 #pragma unroll
-            for (int r = 0; r < P; ++r) st(tr, o + r * NS, v[u][r]);
+            for (int r = 0; r < P; ++r) st(tr, o + r * NS, v[u][r], u, r);
         }
     }
 }
@@ -219,37 +317,29 @@ __device__ __forceinline__ void stage(const double2* __restrict__ tw, Ld ld, St 
 // buffer; OUT_LDS: st_out writes it (each then needs a barrier between the
 // stage's reads and writes).  The caller provides the barrier before the
 // call when LDS was in use; the call ends without a trailing barrier.
-template <int N, int NT, int TPB, bool INV, bool IN_LDS, bool OUT_LDS, int NS, class LdIn, class StOut>
-__device__ __forceinline__ void dft_rest(double2* lds, const double2* __restrict__ tw, LdIn ld_in, StOut st_out)
+template <int N, int NT, int TPB, int MAXR, bool INV, bool IN_LDS, bool OUT_LDS, int NS, int SI, class LdIn, class StOut>
+__device__ __forceinline__ void dft_rest(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in, StOut st_out)
 {
     using Lay = LdsLayout<N, NT>;
-    constexpr int P = Radix<N>::of(NS);
+    constexpr int P = Radix<N, MAXR>::of(NS);
+    auto lds_ld = [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; };
+    auto lds_st = [&](int tr, int idx, double2 v, int, int) { lds[Lay::at(tr, idx)] = v; };
     if constexpr (NS * P >= N) {
-        // last stage
-        if constexpr (NS == 1) {
-            stage<N, NT, TPB, P, NS, INV, IN_LDS && OUT_LDS>(tw, ld_in, st_out);
-        } else {
-            stage<N, NT, TPB, P, NS, INV, OUT_LDS>(
-                tw, [&](int tr, int idx) { return lds[Lay::at(tr, idx)]; }, st_out);
-        }
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, IN_LDS && OUT_LDS>(t, ld_in, st_out);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, OUT_LDS>(t, lds_ld, st_out);
     } else {
-        if constexpr (NS == 1) {
-            stage<N, NT, TPB, P, NS, INV, IN_LDS>(
-                tw, ld_in, [&](int tr, int idx, double2 v) { lds[Lay::at(tr, idx)] = v; });
-        } else {
-            stage<N, NT, TPB, P, NS, INV, true>(
-                tw, [&](int tr, int idx) { return lds[Lay::at(tr, idx)]; },
-                [&](int tr, int idx, double2 v) { lds[Lay::at(tr, idx)] = v; });
-        }
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, IN_LDS>(t, ld_in, lds_st);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, true>(t, lds_ld, lds_st);
         __syncthreads();
-        dft_rest<N, NT, TPB, INV, IN_LDS, OUT_LDS, NS * P>(lds, tw, ld_in, st_out);
+        dft_rest<N, NT, TPB, MAXR, INV, IN_LDS, OUT_LDS, NS * P, SI + 1>(lds, t, ld_in, st_out);
     }
 }
 
-template <int N, int NT, int TPB, bool INV, bool IN_LDS = false, bool OUT_LDS = false, class LdIn, class StOut>
-__device__ __forceinline__ void dft(double2* lds, const double2* __restrict__ tw, LdIn ld_in, StOut st_out)
+template <int N, int NT, int TPB, bool INV, bool IN_LDS = false, bool OUT_LDS = false, int MAXR, class LdIn,
+          class StOut>
+__device__ __forceinline__ void dft(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in, StOut st_out)
 {
-    dft_rest<N, NT, TPB, INV, IN_LDS, OUT_LDS, 1>(lds, tw, ld_in, st_out);
+    dft_rest<N, NT, TPB, MAXR, INV, IN_LDS, OUT_LDS, 1, 0>(lds, t, ld_in, st_out);
 }
 
 }  // namespace pleng
