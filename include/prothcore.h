@@ -31,7 +31,7 @@ extern "C" {
 #define PL_ERR_STATE (-4)    /* device carry did not settle (internal bug)  */
 #define PL_ERR_IO (-5)       /* checkpoint file could not be read/written   */
 
-#define PL_ABI_VERSION 1
+#define PL_ABI_VERSION 2
 
 typedef struct pl_ctx pl_ctx;
 
@@ -58,10 +58,20 @@ int pl_device_count(int* count);
  * and whose limb layout always holds the residue (no "no-fit" band). */
 int pl_plan_fft_len(uint32_t n, uint64_t k, uint32_t* fft_len);
 
+/* Measured FFT length (replaces auto_tune_fft_len, P/dcore.cpp:20-81, and
+ * keeps its probe: limbs all ones with limbs[0] = k|1, 100 squarings, max
+ * round-off against `err_guard`; <= 0 selects the reference's 0.30).  Tries
+ * the rule's length, the next shorter power of two the round-off limits
+ * allow, and the next longer one; times 256 squarings at each length that
+ * passes and picks the fastest.  *fft_len gets the choice, *max_err (nullable)
+ * its probe round-off, *us_per_squaring (nullable) its measured time. */
+int pl_autotune_fft_len(uint32_t n, uint64_t k, int device, double err_guard, uint32_t* fft_len, double* max_err,
+                        double* us_per_squaring);
+
 /* Context create/destroy (replaces fft_init / fft_destroy,
  * P/fft_engine.h:10-11).  fft_len = 0 picks pl_plan_fft_len's length.
  * The residue starts at 0.  The environment variable PL_FUSED=0 disables
- * the fused outer pass (4 kernel passes per squaring instead of 2). */
+ * the split outer pass (5 kernels per squaring instead of 3). */
 int pl_ctx_create(uint64_t k, uint32_t n, uint32_t fft_len, int device, pl_ctx** out);
 void pl_ctx_destroy(pl_ctx* ctx);
 int pl_ctx_info(const pl_ctx* ctx, pl_info* out);
@@ -123,6 +133,10 @@ int pl_gate_rejects(uint64_t k, uint32_t n);
  * 2..99, with Jacobi(b, N) = -1; 0 when none qualifies. */
 uint64_t pl_choose_base(uint64_t k, uint32_t n);
 
+/* Progress report of pl_proth_test: `done` squarings of `total` (n-1)
+ * applied, max round-off so far. */
+typedef void (*pl_progress_fn)(void* user, uint64_t done, uint64_t total, double max_err);
+
 typedef struct pl_test_args {
     uint64_t k;
     uint32_t n;
@@ -132,6 +146,15 @@ typedef struct pl_test_args {
     uint64_t checkpoint_every;  /* squarings between checkpoints (0: 10000)     */
     uint64_t stop_after;        /* nonzero: stop after that many squarings      */
     double err_guard;           /* <= 0: 0.40                                   */
+    int report_step;            /* > 0: call `progress` each time the percentage
+                                   (i+1)*100/(n-1) reaches a new multiple of it
+                                   (the report rule of P/proth_logic.cpp:541-548) */
+    pl_progress_fn progress;    /* nullable                                     */
+    void* progress_user;
+    uint64_t* final_limbs;      /* nullable: the final canonical residue as LE
+                                   u64 limbs, zero-filled (the reference leaves
+                                   it in `limbs`, P/proth_logic.cpp:550-577)    */
+    size_t final_n_limbs;
 } pl_test_args;
 
 typedef struct pl_test_result {

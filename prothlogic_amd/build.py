@@ -58,10 +58,16 @@ def build_host() -> Path | None:
     out = PKG / "libprothhost.so"
     tmp = out.with_suffix(".so.tmp")
     cxx = shutil.which("g++") or "g++"
-    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{INCLUDE}", f"-I{HOST}", "-o", tmp, *srcs,
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", f"-I{HOST}", "-o", tmp, *srcs,
            f"-L{PKG}", "-lprothcore", "-Wl,-rpath,$ORIGIN"]
     _run(cmd)
     os.replace(tmp, out)
+    # the mirror's self-test program (tests/test_host_mirror.py runs it)
+    exe = PKG / "prothhost_selftest"
+    tmp = PKG / "prothhost_selftest.tmp"
+    _run([cxx, "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{HOST}", "-o", tmp, HOST / "selftest.cpp",
+          f"-L{PKG}", "-lprothhost", "-lprothcore", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, exe)
     return out
 
 

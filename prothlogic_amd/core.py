@@ -35,11 +35,16 @@ class PlInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64), ("passes", ctypes.c_uint32)]
 
 
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double)
+
+
 class PlTestArgs(ctypes.Structure):
     _fields_ = [("k", ctypes.c_uint64), ("n", ctypes.c_uint32), ("fft_len", ctypes.c_uint32),
                 ("device", ctypes.c_int), ("checkpoint_path", ctypes.c_char_p),
                 ("checkpoint_every", ctypes.c_uint64), ("stop_after", ctypes.c_uint64),
-                ("err_guard", ctypes.c_double)]
+                ("err_guard", ctypes.c_double), ("report_step", ctypes.c_int), ("progress", PROGRESS_FN),
+                ("progress_user", ctypes.c_void_p), ("final_limbs", ctypes.POINTER(ctypes.c_uint64)),
+                ("final_n_limbs", ctypes.c_size_t)]
 
 
 class PlTestResult(ctypes.Structure):
@@ -83,6 +88,7 @@ def lib() -> ctypes.CDLL:
         "pl_fft_square_karatsuba": ([vp, _u64p, sz, ctypes.c_uint, u64, _dp], i32),
         "pl_kernel_times": ([vp, i32, _dp, _u64p], i32),
         "pl_copy_probe": ([i32, u64, i32, _dp], i32),
+        "pl_autotune_fft_len": ([u32, u64, i32, dbl, ctypes.POINTER(u32), _dp, _dp], i32),
         "pl_gate_rejects": ([u64, u32], i32),
         "pl_choose_base": ([u64, u32], u64),
         "pl_proth_test": ([ctypes.POINTER(PlTestArgs), ctypes.POINTER(PlTestResult)], i32),
@@ -227,13 +233,32 @@ class Context:
 
 
 def proth_test(k: int, n: int, fft_len: int = 0, device: int = 0, checkpoint_path: str | None = None,
-               checkpoint_every: int = 0, stop_after: int = 0, err_guard: float = 0.0) -> dict:
+               checkpoint_every: int = 0, stop_after: int = 0, err_guard: float = 0.0, report_step: int = 0,
+               progress=None, final_n_limbs: int = 0) -> dict:
+    """Full Proth test on the device (pl_proth_test).  progress(done, total,
+    max_err) is called at every report_step percent; final_n_limbs > 0
+    returns the final residue as `residue`."""
+    cb = PROGRESS_FN(lambda _u, done, total, err: progress(done, total, err)) if progress else PROGRESS_FN()
+    fl = (ctypes.c_uint64 * final_n_limbs)() if final_n_limbs else None
     a = PlTestArgs(k, n, fft_len, device, checkpoint_path.encode() if checkpoint_path else None,
-                   checkpoint_every, stop_after, err_guard)
+                   checkpoint_every, stop_after, err_guard, report_step, cb, None,
+                   ctypes.cast(fl, ctypes.POINTER(ctypes.c_uint64)) if fl is not None else None, final_n_limbs)
     r = PlTestResult()
     _check(lib().pl_proth_test(ctypes.byref(a), ctypes.byref(r)))
-    return dict(verdict=r.verdict, gated=bool(r.gated), base=r.base, res64=r.res64, iters_done=r.iters_done,
-                resumed_at=r.resumed_at, max_err=r.max_err, fft_len=r.fft_len)
+    out = dict(verdict=r.verdict, gated=bool(r.gated), base=r.base, res64=r.res64, iters_done=r.iters_done,
+               resumed_at=r.resumed_at, max_err=r.max_err, fft_len=r.fft_len)
+    if fl is not None:
+        out["residue"] = _int(fl)
+    return out
+
+
+def autotune_fft_len(n: int, k: int, device: int = 0, err_guard: float = 0.0) -> dict:
+    """Measured FFT length (pl_autotune_fft_len)."""
+    L = ctypes.c_uint32(0)
+    e = ctypes.c_double(0.0)
+    us = ctypes.c_double(0.0)
+    _check(lib().pl_autotune_fft_len(n, k, device, err_guard, ctypes.byref(L), ctypes.byref(e), ctypes.byref(us)))
+    return dict(fft_len=L.value, max_err=e.value, us_per_squaring=us.value)
 
 
 def save_checkpoint(path: str, k: int, n: int, base: int, next_iter: int, x: int, nl: int):

@@ -281,6 +281,63 @@ int pl_copy_probe(int device, uint64_t bytes, int reps, double* ms)
     });
 }
 
+int pl_autotune_fft_len(uint32_t n, uint64_t k, int device, double err_guard, uint32_t* fft_len, double* max_err,
+                        double* us_per_squaring)
+{
+    if (!fft_len) return fail(PL_ERR_ARG, "fft_len is NULL");
+    uint32_t L0 = 0;
+    int rc = pl_plan_fft_len(n, k, &L0);
+    if (rc) return rc;
+    const double guard = err_guard > 0 ? err_guard : 0.30;
+    uint32_t best = 0;
+    double best_us = 0, best_err = 0;
+    std::string last_why = "no length passed the round-off probe";
+    for (uint32_t L : { L0 / 2, L0, L0 * 2 }) {
+        if (L < 1024) continue;
+        pl_ctx* raw = nullptr;
+        if (pl_ctx_create(k, n, L, device, &raw) != PL_OK) {
+            last_why = g_err;
+            continue;  // this length is not allowed for n (too few or too many bits per digit)
+        }
+        std::unique_ptr<pl_ctx> c(raw);
+        // the reference probe: all-ones limbs with limbs[0] = k|1 (P/dcore.cpp:52-56)
+        std::vector<uint64_t> v((n + 63) / 64, ~0ull);
+        v[0] = k | 1;
+        double e = 0;
+        if (pl_set_residue(c.get(), v.data(), v.size()) != PL_OK) return PL_ERR_HIP;
+        rc = pl_square(c.get(), 100, 1.0, &e, nullptr);
+        if (rc == PL_ERR_STATE || e > guard) continue;
+        if (rc) return rc;
+        const int reps = 256;
+        hipEvent_t t0, t1;
+        float ms = 0;
+        rc = guarded([&] {
+            pleng::check(hipEventCreate(&t0), "hipEventCreate");
+            pleng::check(hipEventCreate(&t1), "hipEventCreate");
+            pleng::check(hipEventRecord(t0, c->eng->stream()), "hipEventRecord");
+            c->eng->square(reps, nullptr);
+            pleng::check(hipEventRecord(t1, c->eng->stream()), "hipEventRecord");
+            pleng::check(hipEventSynchronize(t1), "hipEventSynchronize");
+            pleng::check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
+            (void)hipEventDestroy(t0);
+            (void)hipEventDestroy(t1);
+            return PL_OK;
+        });
+        if (rc) return rc;
+        const double us = 1000.0 * ms / reps;
+        if (!best || us < best_us) {
+            best = L;
+            best_us = us;
+            best_err = e;
+        }
+    }
+    if (!best) return fail(PL_ERR_ROUNDOFF, last_why);
+    *fft_len = best;
+    if (max_err) *max_err = best_err;
+    if (us_per_squaring) *us_per_squaring = best_us;
+    return PL_OK;
+}
+
 int pl_gate_rejects(uint64_t k, uint32_t n)
 {
     static const uint64_t ps[] = { 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67 };
@@ -449,7 +506,7 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
                 int lr = pl_load_checkpoint(args->checkpoint_path, k, n, &b, &nx, buf.data(), nl);
                 if (lr != PL_OK) return lr;  // a corrupt or foreign file is an error, not a silent restart
                 if (nx > total) return fail(PL_ERR_IO, "checkpoint iteration beyond n-1");
-                ctx->set_value(plbn::Limbs(buf.begin(), buf.end()));
+                ctx->set_value(plbn::mod_N(*ctx->M, plbn::Limbs(buf.begin(), buf.end())));
                 next = nx;
                 resumed = true;
             }
@@ -475,8 +532,20 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
             }
         }
         out->resumed_at = next;
+        // progress points: squaring count t at which floor(t*100/total) first
+        // reaches the next multiple of report_step (P/proth_logic.cpp:541-548)
+        const int step_pct = args->progress && args->report_step > 0 ? args->report_step : 0;
+        uint64_t next_pct = 0, report_at = total + 1;
+        auto arm = [&](uint64_t done) {
+            if (!step_pct) return;
+            const uint64_t pct = done * 100 / total;
+            next_pct = (pct / (uint64_t)step_pct + 1) * (uint64_t)step_pct;
+            report_at = next_pct > 100 ? total + 1 : (next_pct * total + 99) / 100;
+        };
+        arm(next);
         while (next < total) {
             uint64_t chunk = std::min<uint64_t>(every - (next % every), total - next);
+            if (report_at > next && report_at - next < chunk) chunk = report_at - next;
             if (args->stop_after && out->iters_done + chunk > args->stop_after)
                 chunk = args->stop_after - out->iters_done;
             if (chunk == 0) break;
@@ -487,6 +556,10 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
             if (e > guard) return fail(PL_ERR_ROUNDOFF, "round-off " + std::to_string(e) + " above guard");
             next += chunk;
             out->iters_done += chunk;
+            if (step_pct && next >= report_at) {
+                args->progress(args->progress_user, next, total, out->max_err);
+                arm(next);
+            }
             if (args->checkpoint_path && next < total && (next % every == 0 || args->stop_after)) {
                 plbn::Limbs x = ctx->value();
                 int s = pl_save_checkpoint(args->checkpoint_path, k, n, a, next, x.data(), x.size());
@@ -496,6 +569,11 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
         }
         plbn::Limbs x = ctx->value();
         out->res64 = plbn::low64(x);
+        if (args->final_limbs) {
+            for (size_t i = 0; i < args->final_n_limbs; ++i) args->final_limbs[i] = i < x.size() ? x[i] : 0;
+            for (size_t i = args->final_n_limbs; i < x.size(); ++i)
+                if (x[i]) return fail(PL_ERR_ARG, "final_n_limbs cannot hold the residue");
+        }
         if (next < total) {
             out->verdict = -1;
             return PL_OK;

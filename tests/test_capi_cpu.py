@@ -24,7 +24,8 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 15
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.pl_abi_version() == 1
+    declared = int(re.search(r"#define PL_ABI_VERSION (\d+)", (ROOT / "include" / "prothcore.h").read_text()).group(1))
+    assert lib.pl_abi_version() == declared
 
 
 def test_gate_matches_closed_form():
