@@ -155,6 +155,14 @@ typedef struct pl_test_args {
                                    u64 limbs, zero-filled (the reference leaves
                                    it in `limbs`, P/proth_logic.cpp:550-577)    */
     size_t final_n_limbs;
+    uint32_t gerbicz_block;     /* > 0: Gerbicz check with blocks of this many
+                                   squarings (planned fault tolerance of the
+                                   reference, README.md:19; no counterpart in
+                                   its code).  0: off                          */
+    uint32_t gerbicz_blocks;    /* blocks per verified epoch (0: 64)            */
+    uint64_t inject_error_at;   /* tests only: add 1 to the residue once, after
+                                   the first block that passes this squaring
+                                   count (0: never); the check must catch it   */
 } pl_test_args;
 
 typedef struct pl_test_result {
@@ -166,13 +174,24 @@ typedef struct pl_test_result {
     uint64_t resumed_at;/* first squaring index run in this call               */
     double max_err;     /* max round-off over the run                          */
     uint32_t fft_len;
+    uint32_t gerbicz_checks;    /* epochs verified or rejected                 */
+    uint32_t gerbicz_rollbacks; /* epochs rejected and recomputed              */
 } pl_test_result;
 
 /* Full Proth test a^((N-1)/2) mod N with the residue resident on the device
  * (replaces execute_proth_test_fft, P/proth_logic.h:12 /
  * P/proth_logic.cpp:384-580, and its checkpoint call sites :449-451,
  * :536-538, :569, :576).  The checkpoint stores the next squaring index, so
- * a resumed run applies exactly n-1 squarings in total. */
+ * a resumed run applies exactly n-1 squarings in total.
+ *
+ * With gerbicz_block = B > 0 the squarings run in epochs of M blocks: after
+ * each block the running product d of the block-end residues u_0..u_j is
+ * updated on the device (d <- d*u_j), and at the epoch's end
+ * d_(M-1)^(2^B) * u_0 must equal d_M (u_j^(2^B) = u_(j+1) for an error-free
+ * chain).  A verified epoch end is the only state checkpointed and the
+ * rollback point: a failed check recomputes the epoch from there (three
+ * failures in a row end the test with PL_ERR_STATE).  Cost: B squarings and
+ * M+1 products per epoch of M*B squarings. */
 int pl_proth_test(const pl_test_args* args, pl_test_result* out);
 
 /* Checkpoint files.  Format v2 (written): 64-byte header

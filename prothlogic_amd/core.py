@@ -44,13 +44,15 @@ class PlTestArgs(ctypes.Structure):
                 ("checkpoint_every", ctypes.c_uint64), ("stop_after", ctypes.c_uint64),
                 ("err_guard", ctypes.c_double), ("report_step", ctypes.c_int), ("progress", PROGRESS_FN),
                 ("progress_user", ctypes.c_void_p), ("final_limbs", ctypes.POINTER(ctypes.c_uint64)),
-                ("final_n_limbs", ctypes.c_size_t)]
+                ("final_n_limbs", ctypes.c_size_t), ("gerbicz_block", ctypes.c_uint32),
+                ("gerbicz_blocks", ctypes.c_uint32), ("inject_error_at", ctypes.c_uint64)]
 
 
 class PlTestResult(ctypes.Structure):
     _fields_ = [("verdict", ctypes.c_int), ("gated", ctypes.c_int), ("base", ctypes.c_uint64),
                 ("res64", ctypes.c_uint64), ("iters_done", ctypes.c_uint64), ("resumed_at", ctypes.c_uint64),
-                ("max_err", ctypes.c_double), ("fft_len", ctypes.c_uint32)]
+                ("max_err", ctypes.c_double), ("fft_len", ctypes.c_uint32), ("gerbicz_checks", ctypes.c_uint32),
+                ("gerbicz_rollbacks", ctypes.c_uint32)]
 
 
 class PoolResult(ctypes.Structure):
@@ -234,7 +236,8 @@ class Context:
 
 def proth_test(k: int, n: int, fft_len: int = 0, device: int = 0, checkpoint_path: str | None = None,
                checkpoint_every: int = 0, stop_after: int = 0, err_guard: float = 0.0, report_step: int = 0,
-               progress=None, final_n_limbs: int = 0) -> dict:
+               progress=None, final_n_limbs: int = 0, gerbicz_block: int = 0, gerbicz_blocks: int = 0,
+               inject_error_at: int = 0) -> dict:
     """Full Proth test on the device (pl_proth_test).  progress(done, total,
     max_err) is called at every report_step percent; final_n_limbs > 0
     returns the final residue as `residue`."""
@@ -242,11 +245,13 @@ def proth_test(k: int, n: int, fft_len: int = 0, device: int = 0, checkpoint_pat
     fl = (ctypes.c_uint64 * final_n_limbs)() if final_n_limbs else None
     a = PlTestArgs(k, n, fft_len, device, checkpoint_path.encode() if checkpoint_path else None,
                    checkpoint_every, stop_after, err_guard, report_step, cb, None,
-                   ctypes.cast(fl, ctypes.POINTER(ctypes.c_uint64)) if fl is not None else None, final_n_limbs)
+                   ctypes.cast(fl, ctypes.POINTER(ctypes.c_uint64)) if fl is not None else None, final_n_limbs,
+                   gerbicz_block, gerbicz_blocks, inject_error_at)
     r = PlTestResult()
     _check(lib().pl_proth_test(ctypes.byref(a), ctypes.byref(r)))
     out = dict(verdict=r.verdict, gated=bool(r.gated), base=r.base, res64=r.res64, iters_done=r.iters_done,
-               resumed_at=r.resumed_at, max_err=r.max_err, fft_len=r.fft_len)
+               resumed_at=r.resumed_at, max_err=r.max_err, fft_len=r.fft_len, gerbicz_checks=r.gerbicz_checks,
+               gerbicz_rollbacks=r.gerbicz_rollbacks)
     if fl is not None:
         out["residue"] = _int(fl)
     return out

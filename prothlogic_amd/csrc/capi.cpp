@@ -113,7 +113,40 @@ struct pl_ctx {
     }
     plbn::Limbs value() { return plbn::from_mont(*M, mont_value()); }
     void set_value(const plbn::Limbs& x) { set_mont_value(plbn::to_mont(*M, x)); }
+    plbn::Limbs mont_of(uint32_t slot)  // canonical Montgomery form of any slot
+    {
+        std::vector<int32_t> d;
+        int64_t xhi = 0;
+        eng->download(slot, d, xhi);
+        return plbn::from_digits(*M, plan.L, d, xhi);
+    }
 };
+
+namespace {
+
+int create_ctx(uint64_t k, uint32_t n, uint32_t fft_len, int device, uint32_t capacity, pl_ctx** out)
+{
+    if (!out) return fail(PL_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    return guarded([&] {
+        auto c = std::make_unique<pl_ctx>();
+        std::string why;
+        if (!pleng::choose_plan(n, k, fft_len, c->plan, why)) return fail(PL_ERR_ARG, why);
+        int nd = 0;
+        if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return fail(PL_ERR_HIP, "no HIP device visible");
+        if (device < 0 || device >= nd) return fail(PL_ERR_ARG, "device index out of range");
+        c->device = device;
+        c->M = std::make_unique<plbn::Modulus>(n, k);
+        const char* fz = std::getenv("PL_FUSED");
+        const bool allow_split = !(fz && fz[0] == '0');
+        if (!pleng::choose_layout(c->plan, (double)n / c->plan.L, k, 1, allow_split, why)) return fail(PL_ERR_ARG, why);
+        c->eng = std::make_unique<pleng::Group>(c->plan, capacity, device);
+        *out = c.release();
+        return PL_OK;
+    });
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -142,24 +175,7 @@ int pl_plan_fft_len(uint32_t n, uint64_t k, uint32_t* fft_len)
 
 int pl_ctx_create(uint64_t k, uint32_t n, uint32_t fft_len, int device, pl_ctx** out)
 {
-    if (!out) return fail(PL_ERR_ARG, "out is NULL");
-    *out = nullptr;
-    return guarded([&] {
-        auto c = std::make_unique<pl_ctx>();
-        std::string why;
-        if (!pleng::choose_plan(n, k, fft_len, c->plan, why)) return fail(PL_ERR_ARG, why);
-        int nd = 0;
-        if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return fail(PL_ERR_HIP, "no HIP device visible");
-        if (device < 0 || device >= nd) return fail(PL_ERR_ARG, "device index out of range");
-        c->device = device;
-        c->M = std::make_unique<plbn::Modulus>(n, k);
-        const char* fz = std::getenv("PL_FUSED");
-        const bool allow_split = !(fz && fz[0] == '0');
-        if (!pleng::choose_layout(c->plan, (double)n / c->plan.L, k, 1, allow_split, why)) return fail(PL_ERR_ARG, why);
-        c->eng = std::make_unique<pleng::Group>(c->plan, 1, device);
-        *out = c.release();
-        return PL_OK;
-    });
+    return create_ctx(k, n, fft_len, device, 1, out);
 }
 
 void pl_ctx_destroy(pl_ctx* ctx) { delete ctx; }
@@ -485,8 +501,12 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
     }
     const uint64_t a = pl_choose_base(k, n);
     out->base = a;
+    // slots: the residue, plus for the Gerbicz check the running product,
+    // the epoch's first residue, the check value and a scratch copy
+    enum { kX = 0, kD = 1, kU0 = 2, kCk = 3, kT = 4 };
+    const bool gerbicz = args->gerbicz_block > 0;
     pl_ctx* raw = nullptr;
-    int rc = pl_ctx_create(k, n, args->fft_len, args->device, &raw);
+    int rc = create_ctx(k, n, args->fft_len, args->device, gerbicz ? 5 : 1, &raw);
     if (rc) return rc;
     std::unique_ptr<pl_ctx> ctx(raw);
     out->fft_len = ctx->plan.L;
@@ -543,7 +563,82 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
             report_at = next_pct > 100 ? total + 1 : (next_pct * total + 99) / 100;
         };
         arm(next);
-        while (next < total) {
+        // test hook: a one-unit error in the residue, once
+        bool injected = false;
+        auto maybe_inject = [&] {
+            if (!args->inject_error_at || injected || next <= args->inject_error_at) return;
+            plbn::Limbs y = ctx->mont_value();
+            plbn::Limbs one(y.size(), 0);
+            one[0] = 1;
+            plbn::add_in(y, one);
+            ctx->set_mont_value(plbn::mod_N(*ctx->M, y));
+            injected = true;
+        };
+        if (gerbicz) {
+            pleng::Group& G = *ctx->eng;
+            const uint64_t B = args->gerbicz_block;
+            const uint64_t Mb = args->gerbicz_blocks ? args->gerbicz_blocks : 64;
+            const uint64_t end = args->stop_after ? std::min<uint64_t>(total, next + args->stop_after) : total;
+            uint64_t verified = next, last_ckpt = next;
+            int failures = 0;
+            auto start_epoch = [&] {
+                G.copy_slot(kU0, kX);
+                G.copy_slot(kD, kX);
+            };
+            start_epoch();
+            while (next < end) {
+                const uint64_t rem = end - next;
+                const uint64_t b = std::min<uint64_t>(B, rem);
+                const uint64_t m = std::max<uint64_t>(1, std::min<uint64_t>(Mb, rem / b));
+                double e = 0.0;
+                for (uint64_t j = 1; j <= m; ++j) {
+                    e = std::max(e, G.square_slots(kX, 1, b, nullptr));
+                    next += b;
+                    maybe_inject();
+                    if (j == m) G.copy_slot(kCk, kD);  // d_(m-1)
+                    G.copy_slot(kT, kX);
+                    e = std::max(e, G.mul_slot(kD, kT));  // d_j = d_(j-1) * u_j
+                }
+                e = std::max(e, G.square_slots(kCk, 1, b, nullptr));
+                G.copy_slot(kT, kU0);
+                e = std::max(e, G.mul_slot(kCk, kT));  // d_(m-1)^(2^b) * u_0
+                const bool settled = G.status() == 0;
+                if (!settled) G.clear_status();
+                const bool ok = settled && e <= guard && ctx->mont_of(kCk) == ctx->mont_of(kD);
+                ++out->gerbicz_checks;
+                if (!ok) {
+                    ++out->gerbicz_rollbacks;
+                    if (++failures >= 3)
+                        return fail(e > guard ? PL_ERR_ROUNDOFF : PL_ERR_STATE,
+                                    "Gerbicz check failed three times in a row at squaring " + std::to_string(verified));
+                    G.copy_slot(kX, kU0);
+                    next = verified;
+                    start_epoch();
+                    continue;
+                }
+                failures = 0;
+                out->max_err = std::max(out->max_err, e);
+                out->iters_done += next - verified;
+                verified = next;
+                while (step_pct && next >= report_at) {
+                    args->progress(args->progress_user, next, total, out->max_err);
+                    arm(next);
+                }
+                if (args->checkpoint_path && next < total && next - last_ckpt >= every) {
+                    plbn::Limbs x = ctx->value();
+                    int s = pl_save_checkpoint(args->checkpoint_path, k, n, a, next, x.data(), x.size());
+                    if (s) return s;
+                    last_ckpt = next;
+                }
+                if (next < end) start_epoch();
+            }
+            if (args->stop_after && next < total && args->checkpoint_path) {
+                plbn::Limbs x = ctx->value();
+                int s = pl_save_checkpoint(args->checkpoint_path, k, n, a, next, x.data(), x.size());
+                if (s) return s;
+            }
+        }
+        while (next < total && !gerbicz) {
             uint64_t chunk = std::min<uint64_t>(every - (next % every), total - next);
             if (report_at > next && report_at - next < chunk) chunk = report_at - next;
             if (args->stop_after && out->iters_done + chunk > args->stop_after)
@@ -556,6 +651,7 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
             if (e > guard) return fail(PL_ERR_ROUNDOFF, "round-off " + std::to_string(e) + " above guard");
             next += chunk;
             out->iters_done += chunk;
+            maybe_inject();
             if (step_pct && next >= report_at) {
                 args->progress(args->progress_user, next, total, out->max_err);
                 arm(next);

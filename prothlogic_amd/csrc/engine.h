@@ -10,7 +10,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace pleng {
@@ -80,6 +82,14 @@ class Group {
     // from and back to the digit state.  err_trace (iters entries, nullable)
     // gets slot 0's per-iteration max round-off.  Returns the max over all.
     double square(uint64_t iters, double* err_trace);
+    // The same for slots [slot0, slot0 + slots) only (err_trace: slot0's).
+    double square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, double* err_trace);
+
+    // Slot arithmetic in the digit state.  copy_slot: dst <- src.
+    // mul_slot: dst <- dst*src mod N (returns its round-off); src is left
+    // transformed, so copy it first when it is still needed.
+    void copy_slot(uint32_t dst, uint32_t src);
+    double mul_slot(uint32_t dst, uint32_t src);
 
     // Scheduled stepping.  enter: digit state -> first squaring's FFT done;
     // step: carry squaring i (x <- x^2 * a^bit) and start squaring i+1 for
@@ -114,7 +124,7 @@ class Group {
     void enqueue_plain_chunk(int g);
     void enqueue_step_chunk(int g);
     void build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g);
-    double read_err(int g, double* trace, double* slot_err);
+    double read_err(int g, double* trace, double* slot_err, uint32_t trace_slot = 0);
     void enqueue_err_clear();
     void enqueue_err_reduce(uint32_t slot0, uint32_t slots);
 
@@ -139,8 +149,11 @@ class Group {
     DeviceTables* tabs_ = nullptr;
     uint32_t carry_blocks_ = 0, carry_tpb_ = 64;
     std::vector<SlotDesc> desc_host_;
-    hipGraphExec_t plain_exec_ = nullptr, step_exec_ = nullptr;
-    hipGraph_t plain_graph_ = nullptr, step_graph_ = nullptr;
+    hipGraphExec_t step_exec_ = nullptr;
+    hipGraph_t step_graph_ = nullptr;
+    // plain squaring chunks, one graph per slot range (slot0 << 32 | slots)
+    std::map<uint64_t, std::pair<hipGraphExec_t, hipGraph_t>> plain_graphs_;
+    uint32_t sq0_ = 0, sqn_ = 1;
     std::vector<unsigned long long> err_host_;
 };
 

@@ -84,6 +84,7 @@ struct Params {
     long long* runc;        // split outer pass: 3 carry-outs per run (block-major, then row)
     int* dig;               // split outer pass: NT int32 digits per run (same order)
     long long* xhi;
+    const long long* xhi_mul;  // product pass: the other factor's X_hi (nullptr: squaring)
     unsigned long long* err;  // per-block round-off partials: block b of this squaring at err[b*kChunk]
     int* status;
     const SlotDesc* desc;
@@ -308,13 +309,15 @@ __device__ __forceinline__ void ripple(const Params& p, unsigned m0, __int128 c2
     if (rem != 0 || c != 0) atomicOr(p.status, 1);
 }
 
-// Second-fold carry-in of run (0, 0): mult*(c2_top + X_hi^2), and the first
+// Second-fold carry-in of run (0, 0): mult*(c2_top + X_hi^2) -- for a
+// product X_hi*Y_hi, the term the pointwise step leaves out -- and the first
 // fold's carry-in c1_top (T_lo's carry at bit n is part of T_hi).
 __device__ __forceinline__ void top_rule(const Params& p, const long long* top, __int128& c2in, __int128& extra)
 {
     const long long xh = *p.xhi;
+    const long long yh = p.xhi_mul ? *p.xhi_mul : xh;
     c2in = top[0];
-    extra = ((__int128)top[1] + (__int128)xh * xh) * p.mult;
+    extra = ((__int128)top[1] + (__int128)xh * yh) * p.mult;
 }
 
 __device__ __forceinline__ double wave_max(double v)
@@ -381,6 +384,41 @@ __global__ __launch_bounds__(TPB) void k_row(Params pb)
     dft<C, NT, TPB, false, false, true>(
         lds, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
         [&](int tr, int idx, double2 X, int, int) { lds[Lay::at(tr, idx)] = cmul(X, make_double2(X.x, X.y + x2)); });
+    __syncthreads();
+    dft<C, NT, TPB, true, true>(
+        lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; },
+        [&](int tr, int idx, double2 v, int, int) { p.z[base + (size_t)tr * C + idx] = v; });
+}
+
+// Row pass of a product: this slot's row times the same row of another
+// residue in the same column-transformed state (zo, X_hi,o).  Forward
+// C-point DFTs of both, Z <- Zd*(Zo + i*X_hi,o) + i*X_hi,d*Zo (the product
+// of Zd + i*X_hi,d and Zo + i*X_hi,o without its constant -X_hi,d*X_hi,o,
+// which the carry's top rule adds back as X_hi,d*X_hi,o*2^(2n)), inverse
+// DFT into this slot.  For zo == z it is the squaring pass.
+template <int C, int NT, int TPB>
+__global__ __launch_bounds__(TPB) void k_row_mul(Params pb, const double2* __restrict__ zo, const long long* __restrict__ xo_p)
+{
+    const Params p = bind(pb, blockIdx.y);
+    if (!p.active) return;
+    using Lay = LdsLayout<C, NT>;
+    __shared__ double2 lds[Lay::kSlots];
+    __shared__ double2 ldo[Lay::kSlots];
+    StageTw<C, NT, TPB, PL_ROW_RADIX> t;
+    load_stage_tw(t, p.tw);
+    const size_t base = (size_t)blockIdx.x * NT * C;
+    const double2* __restrict__ z = p.z;
+    const double xd = (double)(*p.xhi), xo = (double)(*xo_p);
+    dft<C, NT, TPB, false, false, true>(
+        ldo, t, [&](int tr, int idx, int, int) { return zo[base + (size_t)tr * C + idx]; },
+        [&](int tr, int idx, double2 v, int, int) { ldo[Lay::at(tr, idx)] = v; });
+    __syncthreads();
+    dft<C, NT, TPB, false, false, true>(
+        lds, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
+        [&](int tr, int idx, double2 X, int, int) {
+            const double2 O = ldo[Lay::at(tr, idx)];
+            lds[Lay::at(tr, idx)] = cadd(cmul(X, make_double2(O.x, O.y + xo)), make_double2(-xd * O.y, xd * O.x));
+        });
     __syncthreads();
     dft<C, NT, TPB, true, true>(
         lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; },
@@ -950,6 +988,28 @@ static void dispatch_row(const Params& p, uint32_t slots, hipStream_t st)
     }
 }
 
+template <int C>
+static void launch_row_mul(const Params& p, const double2* zo, const long long* xo, hipStream_t st)
+{
+    constexpr int NT = RowCfg<C>::NT, TPB = RowCfg<C>::TPB;
+    hipLaunchKernelGGL((k_row_mul<C, NT, TPB>), dim3(p.R / NT, 1), dim3(TPB), 0, st, p, zo, xo);
+}
+
+static void dispatch_row_mul(const Params& p, const double2* zo, const long long* xo, hipStream_t st)
+{
+    switch (p.C) {
+    case 32: launch_row_mul<32>(p, zo, xo, st); break;
+    case 64: launch_row_mul<64>(p, zo, xo, st); break;
+    case 128: launch_row_mul<128>(p, zo, xo, st); break;
+    case 256: launch_row_mul<256>(p, zo, xo, st); break;
+    case 512: launch_row_mul<512>(p, zo, xo, st); break;
+    case 1024: launch_row_mul<1024>(p, zo, xo, st); break;
+    case 2048: launch_row_mul<2048>(p, zo, xo, st); break;
+    case 4096: launch_row_mul<4096>(p, zo, xo, st); break;
+    default: throw std::runtime_error("unsupported row length");
+    }
+}
+
 template <int RUN, int TPB>
 static void launch_carry_fixup(const Params& p, unsigned blocks, uint32_t slots, bool carry, bool fixup, hipStream_t st)
 {
@@ -1155,8 +1215,10 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
 
 Group::~Group()
 {
-    if (plain_exec_) (void)hipGraphExecDestroy(plain_exec_);
-    if (plain_graph_) (void)hipGraphDestroy(plain_graph_);
+    for (auto& kv : plain_graphs_) {
+        (void)hipGraphExecDestroy(kv.second.first);
+        (void)hipGraphDestroy(kv.second.second);
+    }
     if (step_exec_) (void)hipGraphExecDestroy(step_exec_);
     if (step_graph_) (void)hipGraphDestroy(step_graph_);
     if (mem_) (void)hipFree(mem_);
@@ -1186,6 +1248,7 @@ Params Group::params(uint32_t slot0, bool check_active, uint32_t err_col) const
     p.runc = runc_;
     p.dig = rdig_;
     p.xhi = xhi_;
+    p.xhi_mul = nullptr;
     p.err = errp_ + err_col;
     p.status = status_;
     p.desc = desc_;
@@ -1291,27 +1354,29 @@ void Group::download(uint32_t slot, std::vector<int32_t>& digits, int64_t& xhi)
 
 // g squarings of every slot from the digit state back to the digit state;
 // iteration i's round-off lands in err column i.
+// g plain squarings of slots [sq0_, sq0_ + sqn_).
 void Group::enqueue_plain_chunk(int g)
 {
-    launch(kFwdCol, params(0, false, 0), cap_);
-    launch(kRow, params(0, false, 0), cap_);
+    const uint32_t s0 = sq0_, ns = sqn_;
+    launch(kFwdCol, params(s0, false, 0), ns);
+    launch(kRow, params(s0, false, 0), ns);
     for (int i = 0; i + 1 < g; ++i) {
-        const Params p = params(0, false, (uint32_t)i);
+        const Params p = params(s0, false, (uint32_t)i);
         if (plan_.nt) {
-            launch(kInvCarry, p, cap_);
-            launch(kAdjFwd, p, cap_);
+            launch(kInvCarry, p, ns);
+            launch(kAdjFwd, p, ns);
         } else {
-            launch(kInvCol, p, cap_);
-            launch(kCarry, p, cap_);
-            launch(kFixup, p, cap_);
-            launch(kFwdCol, p, cap_);
+            launch(kInvCol, p, ns);
+            launch(kCarry, p, ns);
+            launch(kFixup, p, ns);
+            launch(kFwdCol, p, ns);
         }
-        launch(kRow, p, cap_);
+        launch(kRow, p, ns);
     }
-    const Params pl = params(0, false, (uint32_t)(g - 1));
-    launch(kInvCol, pl, cap_);
-    launch(kCarry, pl, cap_);
-    launch(kFixup, pl, cap_);
+    const Params pl = params(s0, false, (uint32_t)(g - 1));
+    launch(kInvCol, pl, ns);
+    launch(kCarry, pl, ns);
+    launch(kFixup, pl, ns);
 }
 
 // g scheduled steps of every active slot; step i's round-off in err column i.
@@ -1359,7 +1424,7 @@ void Group::enqueue_err_reduce(uint32_t slot0, uint32_t slots)
 
 // Reads the chunk's err columns: trace gets slot 0's per-column values,
 // slot_err each slot's max; returns the max over everything.
-double Group::read_err(int g, double* trace, double* slot_err)
+double Group::read_err(int g, double* trace, double* slot_err, uint32_t trace_slot)
 {
     check(hipMemcpyAsync(err_host_.data(), err_, sizeof(unsigned long long) * kChunk * cap_, hipMemcpyDeviceToHost,
                          stream_),
@@ -1371,7 +1436,7 @@ double Group::read_err(int g, double* trace, double* slot_err)
         for (int i = 0; i < g; ++i) {
             double e;
             std::memcpy(&e, &err_host_[(size_t)s * kChunk + i], sizeof e);
-            if (s == 0 && trace) trace[i] = e;
+            if (s == trace_slot && trace) trace[i] = e;
             smax = std::max(smax, e);
         }
         if (slot_err) slot_err[s] = smax;
@@ -1380,26 +1445,78 @@ double Group::read_err(int g, double* trace, double* slot_err)
     return emax;
 }
 
-double Group::square(uint64_t iters, double* err_trace)
+double Group::square(uint64_t iters, double* err_trace) { return square_slots(0, cap_, iters, err_trace); }
+
+double Group::square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, double* err_trace)
 {
+    if (slots == 0 || slot0 + slots > cap_) throw std::invalid_argument("square_slots: slot range out of bounds");
     check(hipSetDevice(device_), "hipSetDevice");
+    sq0_ = slot0;
+    sqn_ = slots;
     double emax = 0.0;
     uint64_t done = 0;
     while (done < iters) {
         const int g = (int)std::min<uint64_t>(kChunk, iters - done);
         if (g == kChunk) {
-            if (!plain_exec_) build_graph(plain_exec_, plain_graph_, false, kChunk);
-            check(hipGraphLaunch(plain_exec_, stream_), "hipGraphLaunch");
+            const uint64_t key = ((uint64_t)slot0 << 32) | slots;
+            auto it = plain_graphs_.find(key);
+            if (it == plain_graphs_.end()) {
+                hipGraphExec_t ex = nullptr;
+                hipGraph_t gr = nullptr;
+                build_graph(ex, gr, false, kChunk);
+                it = plain_graphs_.emplace(key, std::make_pair(ex, gr)).first;
+            }
+            check(hipGraphLaunch(it->second.first, stream_), "hipGraphLaunch");
         } else {
             enqueue_err_clear();
             enqueue_plain_chunk(g);
             enqueue_err_reduce(0, cap_);
             check(hipGetLastError(), "square launch");
         }
-        emax = std::max(emax, read_err(g, err_trace ? err_trace + done : nullptr, nullptr));
+        emax = std::max(emax, read_err(g, err_trace ? err_trace + done : nullptr, nullptr, slot0));
         done += g;
     }
     return emax;
+}
+
+void Group::copy_slot(uint32_t dst, uint32_t src)
+{
+    if (dst >= cap_ || src >= cap_) throw std::invalid_argument("copy_slot: slot out of range");
+    if (dst == src) return;
+    check(hipSetDevice(device_), "hipSetDevice");
+    check(hipMemcpyAsync(z_ + (size_t)dst * plan_.L, z_ + (size_t)src * plan_.L, sizeof(double2) * plan_.L,
+                         hipMemcpyDeviceToDevice, stream_),
+          "copy_slot z");
+    check(hipMemcpyAsync(xhi_ + dst, xhi_ + src, sizeof(long long), hipMemcpyDeviceToDevice, stream_), "copy_slot xhi");
+    check(hipStreamSynchronize(stream_), "copy_slot sync");
+}
+
+// dst <- dst*src mod N: forward column pass of both, the two-operand row
+// pass, and the inverse column pass + carry with the product's top rule.
+double Group::mul_slot(uint32_t dst, uint32_t src)
+{
+    if (dst >= cap_ || src >= cap_ || dst == src) throw std::invalid_argument("mul_slot: bad slots");
+    check(hipSetDevice(device_), "hipSetDevice");
+    {
+        const int cnt = (int)((size_t)kChunk * eb_);
+        hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, errp_ + (size_t)dst * cnt, cnt);
+    }
+    launch(kFwdCol, params(dst, false, 0), 1);
+    launch(kFwdCol, params(src, false, 0), 1);
+    Params p = params(dst, false, 0);
+    dispatch_row_mul(p, z_ + (size_t)src * plan_.L, xhi_ + src, stream_);
+    p.xhi_mul = xhi_ + src;
+    launch(kInvCol, p, 1);
+    launch(kCarry, p, 1);
+    launch(kFixup, p, 1);
+    enqueue_err_reduce(dst, 1);
+    check(hipGetLastError(), "mul_slot");
+    unsigned long long e = 0;
+    check(hipMemcpyAsync(&e, err_ + (size_t)dst * kChunk, sizeof e, hipMemcpyDeviceToHost, stream_), "err D2H");
+    check(hipStreamSynchronize(stream_), "mul_slot sync");
+    double d;
+    std::memcpy(&d, &e, sizeof d);
+    return d;
 }
 
 void Group::enter(uint32_t slot)
@@ -1493,7 +1610,9 @@ void Group::kernel_times(int reps, double* ms)
         seq = { kInvCol, kCarry, kFixup, kFwdCol, kRow };
     const int m = (int)seq.size();
     std::vector<hipEvent_t> ev((size_t)(m + 1) * (size_t)reps + 8);
-    for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+    // no system-scope fence per event: between kernels of one stream it only
+    // adds a cache write-back that the squaring itself does not pay
+    for (auto& e : ev) check(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
     for (int kind = 0; kind < kNumKernels; ++kind) ms[kind] = 0.0;
     // enter: the first squaring's forward transform (timed once, then the
     // forward column pass as the unfused steady state times it)
