@@ -154,6 +154,7 @@ class Group {
     // plain squaring chunks, one graph per slot range (slot0 << 32 | slots)
     std::map<uint64_t, std::pair<hipGraphExec_t, hipGraph_t>> plain_graphs_;
     uint32_t sq0_ = 0, sqn_ = 1;
+    bool uniform_ = true;  // every slot holds the plan's n and k
     std::vector<unsigned long long> err_host_;
 };
 

@@ -94,6 +94,9 @@ struct Params {
     unsigned err_stride;
     unsigned slot0;         // first slot of this launch
     int sched;              // apply per-slot multipliers and the active flag
+    int per_slot;           // slots hold different candidates: read n, k from desc
+                            // (else n, k below are every slot's, and no kernel
+                            // starts with a dependent descriptor load)
     // bound per slot by bind()
     unsigned long long n;
     long long k;
@@ -125,16 +128,18 @@ __device__ __forceinline__ Params bind(const Params& base, unsigned y)
     p.wri += (size_t)slot * base.R;
     p.wcf += (size_t)slot * base.C;
     p.wci += (size_t)slot * base.C;
-    const SlotDesc d = base.desc[slot];
-    p.n = d.n;
-    p.k = d.k;
     p.mult = 1;
     p.active = 1;
-    if (base.sched) {
-        const SlotState st = base.state[slot];
-        p.active = st.active;
-        const unsigned long long i = st.carried + 1;  // squaring being carried
-        if (d.a && i <= d.topbit && ((d.k >> (d.topbit - i)) & 1)) p.mult = d.a;
+    if (base.per_slot || base.sched) {
+        const SlotDesc d = base.desc[slot];
+        p.n = d.n;
+        p.k = d.k;
+        if (base.sched) {
+            const SlotState st = base.state[slot];
+            p.active = st.active;
+            const unsigned long long i = st.carried + 1;  // squaring being carried
+            if (d.a && i <= d.topbit && ((d.k >> (d.topbit - i)) & 1)) p.mult = d.a;
+        }
     }
     return p;
 }
@@ -1258,6 +1263,7 @@ Params Group::params(uint32_t slot0, bool check_active, uint32_t err_col) const
     p.err_stride = kChunk * eb_;
     p.slot0 = slot0;
     p.sched = check_active ? 1 : 0;
+    p.per_slot = uniform_ ? 0 : 1;
     p.n = plan_.n;
     p.k = (long long)plan_.k;
     p.mult = 1;
@@ -1301,6 +1307,16 @@ void Group::set_slot(uint32_t slot, uint32_t n, uint64_t k, uint32_t a, uint64_t
     unsigned top = 0;
     while (top < 63 && (k >> (top + 1))) ++top;
     desc_host_[slot] = SlotDesc{ n, (long long)k, a, top, total };
+    if (uniform_ && (n != plan_.n || k != plan_.k)) {
+        // plain-squaring graphs were captured reading n, k from the launch
+        // parameters; from now on every launch reads each slot's descriptor
+        uniform_ = false;
+        for (auto& kv : plain_graphs_) {
+            (void)hipGraphExecDestroy(kv.second.first);
+            (void)hipGraphDestroy(kv.second.second);
+        }
+        plain_graphs_.clear();
+    }
     check(hipMemcpyAsync(desc_ + slot, &desc_host_[slot], sizeof(SlotDesc), hipMemcpyHostToDevice, stream_),
           "desc upload");
     std::vector<double2> wrf(plan_.R), wri(plan_.R), wcf(plan_.C), wci(plan_.C);
