@@ -102,6 +102,16 @@ def roofline(kt: dict, L: int, nt: int):
                      frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None), kb[dom], per_sq
 
 
+def fp64_rate(L: int, us_per_squaring: float) -> dict:
+    """Achieved FP64 rate of one squaring against the vector peak: about
+    2*5*L*log2(L) flops for the forward and inverse transforms plus 20*L for
+    weights, pointwise square and rounding."""
+    flops = 2 * 5 * L * (L.bit_length() - 1) + 20 * L
+    tf = flops / (us_per_squaring * 1e-6) / 1e12
+    return dict(flops_per_squaring=flops, achieved_tflops=round(tf, 3), peak_tflops=FP64_PEAK_TFLOPS,
+                frac=round(tf / FP64_PEAK_TFLOPS, 4))
+
+
 def measured_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel`, from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a copy kernel of known
@@ -309,7 +319,8 @@ def main():
                               res64_matches_gmp=(None if g3 is None else r3["res64"] == g3),
                               kernel_ms=r3["kernel_ms"], dominant_kernel=dom3, roofline=roof3,
                               kernels_per_squaring=r3["kernels_per_squaring"], bytes_per_squaring=sq3_bytes,
-                              hbm_roofline_us=round(roofline_us, 3), x_of_roofline=round(us3 / roofline_us, 2))
+                              hbm_roofline_us=round(roofline_us, 3), x_of_roofline=round(us3 / roofline_us, 2),
+                              fp64=fp64_rate(r3["fft_len"], us3))
         log(f"[bench] c3: {us3:.2f} us/squaring, max_err={r3['max_err']:.4g}")
 
     if not args.no_batch:
@@ -355,6 +366,7 @@ def main():
         "bytes_per_squaring": sq_bytes,
         "hbm_roofline_us": round(sq_bytes / (HBM_PEAK_GBS * 1e9) * 1e6, 3),
         "roofline": roof,
+        "fp64": fp64_rate(r2["fft_len"], us2),
         "cpu_baseline": cpu,
         **extra,
     }

@@ -110,7 +110,9 @@ int pl_fft_square_karatsuba(pl_ctx* ctx, uint64_t* v, size_t n_limbs, unsigned n
 /* Diagnostics (no counterpart in the reference): average duration in ms of
  * each kernel of one squaring, timed with HIP events on the context's own
  * stream between consecutive kernels of `reps` real squarings run in their
- * steady-state order, into ms[0..6]: ms[0] forward column pass,
+ * steady-state order, less the events' own cost (the excess of their sum
+ * over the same loop timed end to end, in equal parts per steady kernel),
+ * into ms[0..6]: ms[0] forward column pass,
  * ms[1] row pass (forward DFT, pointwise square, inverse DFT), ms[2] inverse
  * column pass, ms[3] flat carry/fold, ms[4] block fix-up, ms[5] inverse
  * column DFT + carry/fold, ms[6] carry-in + forward column DFT (ms[5] and

@@ -108,7 +108,8 @@ class Group {
     // Average duration (ms) of each kernel over all slots, timed with HIP
     // events between consecutive kernels of `reps` real plain squarings on
     // this group's stream (kernels in their steady-state order, each on its
-    // predecessor's output): [0] forward column pass, [1] row pass, [2]
+    // predecessor's output), less the events' own cost (measured on the same
+    // loop timed end to end): [0] forward column pass, [1] row pass, [2]
     // inverse column pass, [3] flat carry, [4] block fix-up, [5] inverse
     // column DFT + carry, [6] carry-in + forward column DFT ([5], [6] are 0
     // without the split outer pass; with it, [0], [2]-[4] come from the one

@@ -887,7 +887,10 @@ bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::strin
     out.k = k;
     out.L = L;
     out.logL = logL;
-    const uint32_t logR = std::max<uint32_t>(logL > 12 ? logL - 12 : 0, std::min<uint32_t>(9, logL / 2));
+    // Column DFTs of at most 256 points (more, smaller column blocks: C/8
+    // workgroups) unless the row DFT would pass 4096 points.  At n = 2^24
+    // (L = 2^20) 256 x 4096 squares in 36 us against 42 us for 512 x 2048.
+    const uint32_t logR = std::max<uint32_t>(logL > 12 ? logL - 12 : 0, std::min<uint32_t>(8, logL / 2));
     out.R = 1u << logR;
     out.C = L / out.R;
     out.s = (logL + 1) / 2;
@@ -1663,6 +1666,33 @@ void Group::kernel_times(int reps, double* ms)
             const size_t i = loop0 + (size_t)r * (size_t)m + (size_t)j;
             ms[seq[(size_t)j]] += span(i, i + 1) / reps;
         }
+    // The same steady loop again with events at its two ends only: the
+    // excess of the per-kernel sum over this per-squaring time is what the
+    // events between kernels cost (about 1 us each); it is taken off each
+    // steady kernel in equal parts.
+    {
+        hipEvent_t a, b;
+        check(hipEventCreateWithFlags(&a, hipEventDisableSystemFence), "hipEventCreate");
+        check(hipEventCreateWithFlags(&b, hipEventDisableSystemFence), "hipEventCreate");
+        launch(kFwdCol, p, cap_);
+        launch(kRow, p, cap_);
+        check(hipEventRecord(a, stream_), "hipEventRecord");
+        for (int r = 0; r < reps; ++r)
+            for (int j = 0; j < m; ++j) launch(seq[(size_t)j], p, cap_);
+        check(hipEventRecord(b, stream_), "hipEventRecord");
+        launch(kInvCol, p, cap_);
+        launch(kCarry, p, cap_);
+        launch(kFixup, p, cap_);
+        check(hipEventSynchronize(b), "hipEventSynchronize");
+        float t = 0;
+        check(hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        double sum = 0;
+        for (int j = 0; j < m; ++j) sum += ms[seq[(size_t)j]];
+        const double per_event = std::max(0.0, (sum - (double)t / reps) / m);
+        for (int j = 0; j < m; ++j) ms[seq[(size_t)j]] = std::max(0.0, ms[seq[(size_t)j]] - per_event);
+    }
     if (plan_.nt) {
         ms[kFwdCol] = span(0, 1);
         ms[kInvCol] = span(tail0, tail0 + 1);
