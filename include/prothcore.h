@@ -210,6 +210,30 @@ int pl_load_checkpoint(const char* path, uint64_t k, uint32_t n, uint64_t* base,
                        uint64_t* limbs, size_t n_limbs);
 
 /* ------------------------------------------------------------------------
+ * Candidate sieve (the step before the test).  A k is removed when some
+ * prime 3 <= p <= pmax divides k*2^n+1 (k == -2^(-n) mod p).  Errors are
+ * described by pl_sieve_last_error().
+ * ---------------------------------------------------------------------- */
+
+/* Replaces execute_full_streaming_sieve (P/presieve_core.h:8-12,
+ * P/presieve_core.cpp:15-88, over init_gpu_sieve .. finalize_gpu_sieve,
+ * P/gpu_engine.h:13-30): the odd k in [k_start, k_end] that survive, in
+ * ascending order.  Writes min(cap, survivors) of them to out_ks and sets
+ * *n_out to the number of survivors (call with cap = 0 to count).  The range
+ * is limited to 2^36 values and pmax to 2^62. */
+int pl_sieve_range(uint64_t k_start, uint64_t k_end, uint32_t n, uint64_t pmax, int device, uint64_t* out_ks,
+                   size_t cap, size_t* n_out);
+
+/* Replaces execute_sparse_streaming_sieve (P/presieve_core.h:14-18,
+ * P/presieve_core.cpp:93-167): keep[i] <- 1 when ks[i] survives every prime
+ * in [pstart, pmax], else 0.  The reference compares k itself, not k mod p,
+ * with the target (P/gpu_engine.cu:397), so it removes almost nothing; this
+ * is the test the reference intends. */
+int pl_sieve_list(const uint64_t* ks, size_t count, uint32_t n, uint64_t pstart, uint64_t pmax, int device,
+                  uint8_t* keep);
+const char* pl_sieve_last_error(void);
+
+/* ------------------------------------------------------------------------
  * Continuous batching.  Replaces the reference's per-thread dispatch of one
  * test per worker (dispatch, P/dcore.cpp:84-193, calling run_task_with_ctx,
  * P/worker.cpp:27-70, for each k of a PROTH_FF_BATCH task :51-67) with one

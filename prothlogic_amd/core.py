@@ -91,6 +91,9 @@ def lib() -> ctypes.CDLL:
         "pl_kernel_times": ([vp, i32, _dp, _u64p], i32),
         "pl_copy_probe": ([i32, u64, i32, _dp], i32),
         "pl_autotune_fft_len": ([u32, u64, i32, dbl, ctypes.POINTER(u32), _dp, _dp], i32),
+        "pl_sieve_range": ([u64, u64, u32, u64, i32, _u64p, sz, ctypes.POINTER(sz)], i32),
+        "pl_sieve_list": ([_u64p, sz, u32, u64, u64, i32, ctypes.POINTER(ctypes.c_uint8)], i32),
+        "pl_sieve_last_error": ([], ctypes.c_char_p),
         "pl_gate_rejects": ([u64, u32], i32),
         "pl_choose_base": ([u64, u32], u64),
         "pl_proth_test": ([ctypes.POINTER(PlTestArgs), ctypes.POINTER(PlTestResult)], i32),
@@ -255,6 +258,28 @@ def proth_test(k: int, n: int, fft_len: int = 0, device: int = 0, checkpoint_pat
     if fl is not None:
         out["residue"] = _int(fl)
     return out
+
+
+def _scheck(rc: int):
+    if rc != PL_OK:
+        raise PlError(rc, lib().pl_sieve_last_error().decode(errors="replace"))
+
+
+def sieve_range(k_start: int, k_end: int, n: int, pmax: int, device: int = 0) -> list:
+    """Odd k in [k_start, k_end] with no prime factor p <= pmax of k*2^n+1."""
+    cap = max(1, (k_end - k_start) // 2 + 1) if k_end >= k_start else 1  # odd k in the range
+    out = (ctypes.c_uint64 * cap)()
+    cnt = ctypes.c_size_t(0)
+    _scheck(lib().pl_sieve_range(k_start, k_end, n, pmax, device, out, cap, ctypes.byref(cnt)))
+    return list(out[:cnt.value])
+
+
+def sieve_list(ks, n: int, pstart: int, pmax: int, device: int = 0) -> list:
+    """keep flags of ks against the primes in [pstart, pmax]."""
+    arr = (ctypes.c_uint64 * max(1, len(ks)))(*ks)
+    keep = (ctypes.c_uint8 * max(1, len(ks)))()
+    _scheck(lib().pl_sieve_list(arr, len(ks), n, pstart, pmax, device, keep))
+    return [bool(v) for v in keep[:len(ks)]]
 
 
 def autotune_fft_len(n: int, k: int, device: int = 0, err_guard: float = 0.0) -> dict:

@@ -37,7 +37,11 @@ SIGNATURES = [
     "std::char_traits<char>, std::allocator<char> > const&)",
     "auto_tune_fft_len(unsigned int, unsigned long)",
     "run_task_with_ctx[abi:cxx11](Task const&, FFTContext&, unsigned long*)",
+    "execute_full_streaming_sieve(unsigned long, unsigned long, unsigned int, unsigned long)",
+    "execute_sparse_streaming_sieve(std::vector<unsigned long, std::allocator<unsigned long> > const&, "
+    "unsigned int, unsigned long, unsigned long)",
 ]
+REF_TREE = Path("/root/reference/prothlogic4.2")
 
 
 def _defined(lib: Path, demangle: bool):
@@ -65,6 +69,22 @@ def test_mirror_mangled_names_equal_reference_build():
         o = {m for m in ours if m.startswith("_Z" + stem)}
         r = {m for m in ref if m.startswith("_Z" + stem)}
         assert o and o == r, (stem, o, r)
+
+
+@pytest.mark.skipif(not (REF_TREE / "presieve_core.h").exists(), reason="reference headers not present on this host")
+def test_sieve_mirror_names_equal_reference_declarations(tmp_path):
+    """The sieve entry points, declared by the reference's own presieve_core.h
+    (compiled here, only to read the symbols it references), are the symbols
+    the mirror defines."""
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "presieve_core.h"\n'
+                   'void* used[] = {(void*)&execute_full_streaming_sieve, (void*)&execute_sparse_streaming_sieve};\n')
+    obj = tmp_path / "use.o"
+    subprocess.run(["g++", "-std=c++17", "-c", f"-I{REF_TREE}", str(src), "-o", str(obj)], check=True)
+    und = subprocess.run(["nm", "-u", str(obj)], capture_output=True, text=True, check=True).stdout.split()
+    wanted = {m for m in und if "streaming_sieve" in m}
+    assert len(wanted) == 2
+    assert wanted <= _defined(HOST_LIB, demangle=False)
 
 
 @pytest.mark.gpu

@@ -219,3 +219,31 @@ def test_restatement_equals_reference_build(k, n, L):
                 assert limbs_to_int(a) == x * x % N
     finally:
         ref.destroy(h)
+
+
+# ---------------------------------------------------------------- sieve oracle
+import sieve_oracle as SO
+
+
+def test_sieve_targets_closed_form():
+    """t_p = -2^(-n) mod p means p divides t_p*2^n + 1."""
+    for n in (1, 2, 17, 1000, 120002):
+        for p in (3, 5, 7, 11, 13, 101, 65537, 999983):
+            t = SO.target(p, n)
+            assert 0 <= t < p and (t * pow(2, n, p) + 1) % p == 0
+
+
+@pytest.mark.parametrize("n,k0,k1,pmax", [(50, 1, 3000, 500), (1000, 1001, 4000, 3000), (64, 10**9, 10**9 + 999, 2000)])
+def test_sieve_range_vs_brute_force(n, k0, k1, pmax):
+    ps = [int(p) for p in SO.primes_upto(pmax) if p >= 3]
+    want = [k for k in range(k0, k1 + 1) if k & 1 and all((k * (1 << n) + 1) % p for p in ps)]
+    assert SO.sieve_range(k0, k1, n, pmax) == want
+
+
+def test_sieve_list_vs_brute_force():
+    rnd = random.Random(9)
+    ks = [rnd.randrange(3, 1 << 40) | 1 for _ in range(300)]
+    n, pstart, pmax = 777, 50, 4000
+    ps = [int(p) for p in SO.primes_upto(pmax) if p >= max(3, pstart)]
+    want = [all((k * (1 << n) + 1) % p for p in ps) for k in ks]
+    assert SO.sieve_list(ks, n, pstart, pmax) == want
