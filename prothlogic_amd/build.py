@@ -68,6 +68,12 @@ def build_host() -> Path | None:
     _run([cxx, "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{HOST}", "-o", tmp, HOST / "selftest.cpp",
           f"-L{PKG}", "-lprothhost", "-lprothcore", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, exe)
+    # the command-line front end
+    exe = PKG / "prothlogic"
+    tmp = PKG / "prothlogic.tmp"
+    _run([cxx, "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{HOST}", "-o", tmp, HOST / "cli.cpp",
+          f"-L{PKG}", "-lprothhost", "-lprothcore", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, exe)
     return out
 
 
