@@ -120,10 +120,13 @@ def _check(rc: int):
 
 
 def _limbs(x: int, nl: int):
-    arr = (ctypes.c_uint64 * nl)()
-    for i in range(nl):
-        arr[i] = (x >> (64 * i)) & 0xFFFFFFFFFFFFFFFF
-    return arr
+    """Low nl 64-bit limbs of x >= 0, least significant first.  One
+    to_bytes pass (linear in the size of x); a shift per limb would be
+    quadratic, minutes at tens of millions of bits."""
+    if x < 0:
+        raise ValueError("residue must be non-negative")
+    x &= (1 << (64 * nl)) - 1
+    return (ctypes.c_uint64 * nl).from_buffer_copy(x.to_bytes(8 * nl, "little"))
 
 
 def _int(arr) -> int:

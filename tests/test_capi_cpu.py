@@ -87,3 +87,21 @@ def test_reads_reference_checkpoint_layout(tmp_path):
     o.save_checkpoint(10000, int_to_limbs(x, 2048), p)   # reference: index of the last applied squaring
     b, it, y = core.load_checkpoint(p, 843, 120002, 2048)
     assert it == 10001 and y == x
+
+
+def test_limb_conversion_is_exact_and_linear():
+    """Residues cross the ABI as little-endian 64-bit limbs; the conversion
+    must truncate to nl limbs and stay linear in the size (60M-bit residues
+    are converted at the longest transform lengths)."""
+    import random
+    import time
+    assert list(core._limbs(5, 2)) == [5, 0]
+    assert list(core._limbs((1 << 70) + 7, 1)) == [7]
+    assert list(core._limbs((1 << 70) + 7, 2)) == [7, 64]
+    with pytest.raises(ValueError):
+        core._limbs(-1, 1)
+    x = random.Random(3).getrandbits(60_000_001)
+    t = time.perf_counter()
+    a = core._limbs(x, 60_000_001 // 64 + 2)
+    assert time.perf_counter() - t < 2.0
+    assert core._int(a) == x

@@ -169,3 +169,23 @@ def test_large_golden_c2_c3(gpu):
                 done = i
                 assert c.res64() == int(r64, 16), (k, n, i)
                 assert e < 0.35
+
+
+# Lengths above 2^18: 2^19 and 2^21 on the split pass, 2^22 on the unfused
+# pass (column DFTs of 1024 points, two butterflies per thread).  At 2^19
+# 66 squarings cross one graph-captured chunk; the longer lengths run a few
+# (GMP, the arbiter, needs about a second per squaring at 60M bits).
+@pytest.mark.parametrize("k,n,L,iters", [(3, 9000001, 1 << 19, 66), (5, 33000001, 1 << 21, 8),
+                                         (3, 60000001, 1 << 22, 5)])
+def test_square_long_lengths_vs_powm(gpu, k, n, L, iters):
+    rnd = random.Random(L)
+    N = k * (1 << n) + 1
+    x = rnd.getrandbits(n) % N
+    print(f"[parity] k={k} n={n} L={L} iters={iters}", flush=True)
+    with gpu.Context(k, n, L) as c:
+        assert c.fft_len == L
+        c.set_residue(x)
+        e = c.square(iters)
+        print(f"[parity] squared, max round-off {e:.4f}; GMP check", flush=True)
+        assert 0 < e < 0.3
+        assert c.res64() == G.powm(x, 1 << iters, N) & M64
