@@ -73,7 +73,7 @@ def time_squarings(core, k, n, steps, warmup):
     info = dict(fft_len=c.fft_len, rows=c.info.rows, cols=c.info.cols, bits_per_digit=round(c.info.bits_per_digit, 4),
                 kernels_per_squaring=c.info.passes)
     c.close()
-    nt = 0 if info["kernels_per_squaring"] == 5 else (8 if kt.get("inv_carry", 0) else 0)
+    nt = 0 if info["kernels_per_squaring"] == 5 else 8
     return dict(seconds=t1 - t0, max_err=emax, res64=r64, kernel_ms=kt, a=a, nt=nt, **info)
 
 
@@ -81,19 +81,30 @@ def kernel_bytes(L: int, nt: int) -> dict:
     """Algorithmic HBM bytes one launch of each kernel moves for a length-L
     residue: complex FP64 points are 16 B (read and/or written once); the
     split outer pass hands 4 B int32 digits and 24 B of run carries per run
-    of nt digits from its first to its second kernel."""
+    of nt digits from its first to its second kernel; the one-launch outer
+    pass reads and writes the points and writes and reads the 24 B of run
+    carries."""
     run = 24.0 / nt if nt else 0.0
     return {"fwd_col": 32 * L, "row_square": 32 * L, "inv_col": 32 * L, "carry": 32 * L,
-            "inv_carry": int((16 + 4 + run) * L), "adjust_fwd": int((4 + run + 16) * L)}
+            "inv_carry": int((16 + 4 + run) * L), "adjust_fwd": int((4 + run + 16) * L),
+            "carry_turn": int((32 + 2 * run) * L)}
 
 
+TURN = ("carry_turn", "row_square")
 STEADY = ("inv_carry", "adjust_fwd", "row_square")
 UNFUSED = ("inv_col", "carry", "fwd_col", "row_square")
 
 
+def steady_kernels(kt: dict, nt: int):
+    """The kernels of one steady-state squaring, as the context ran them."""
+    if not nt:
+        return UNFUSED
+    return TURN if kt.get("carry_turn", 0) else STEADY
+
+
 def roofline(kt: dict, L: int, nt: int):
     """Dominant steady-state kernel's achieved algorithmic GB/s vs the HBM peak."""
-    names = STEADY if nt else UNFUSED
+    names = steady_kernels(kt, nt)
     kb = kernel_bytes(L, nt)
     dom = max(names, key=lambda k: kt[k])
     achieved = kb[dom] / (kt[dom] * 1e-3) / 1e9

@@ -31,7 +31,7 @@ extern "C" {
 #define PL_ERR_STATE (-4)    /* device carry did not settle (internal bug)  */
 #define PL_ERR_IO (-5)       /* checkpoint file could not be read/written   */
 
-#define PL_ABI_VERSION 2
+#define PL_ABI_VERSION 3
 
 typedef struct pl_ctx pl_ctx;
 
@@ -43,8 +43,9 @@ typedef struct pl_info {
     uint32_t cols;         /* C: contiguous DFT length                      */
     double bits_per_digit; /* n / L                                         */
     uint64_t device_bytes; /* HBM held by the context                       */
-    uint32_t passes;       /* kernels per steady-state squaring: 3 with the
-                              split outer pass, 5 without                    */
+    uint32_t passes;       /* kernels per steady-state squaring: 2 with the
+                              one-launch outer pass, 3 with it in two
+                              launches (PL_CARRY_TURN=0), 5 unfused          */
 } pl_info;
 
 /* Thread-local description of the last error (never NULL). */
@@ -112,11 +113,12 @@ int pl_fft_square_karatsuba(pl_ctx* ctx, uint64_t* v, size_t n_limbs, unsigned n
  * stream between consecutive kernels of `reps` real squarings run in their
  * steady-state order, less the events' own cost (the excess of their sum
  * over the same loop timed end to end, in equal parts per steady kernel),
- * into ms[0..6]: ms[0] forward column pass,
+ * into ms[0..7]: ms[0] forward column pass,
  * ms[1] row pass (forward DFT, pointwise square, inverse DFT), ms[2] inverse
  * column pass, ms[3] flat carry/fold, ms[4] block fix-up, ms[5] inverse
- * column DFT + carry/fold, ms[6] carry-in + forward column DFT (ms[5] and
- * ms[6] are 0 when the context does not use the split outer pass).  The
+ * column DFT + carry/fold, ms[6] carry-in + forward column DFT, ms[7] the
+ * one-launch outer pass (inverse column DFT, carry/fold, carry hand-off,
+ * forward column DFT).  Kernels the context does not run read 0.  The
  * residue is preserved.
  * *traffic_bytes (nullable) gets the algorithmic HBM bytes one launch of
  * each full-array kernel moves (read + write of the L complex points). */

@@ -223,12 +223,12 @@ class Context:
         _check(lib().pl_is_n_minus_1(self.h, ctypes.byref(r)))
         return bool(r.value)
 
-    KERNELS = ("fwd_col", "row_square", "inv_col", "carry", "fixup", "inv_carry", "adjust_fwd")
+    KERNELS = ("fwd_col", "row_square", "inv_col", "carry", "fixup", "inv_carry", "adjust_fwd", "carry_turn")
 
     def kernel_times(self, reps: int = 20):
         """{kernel: ms per launch} timed with HIP events on the context's stream,
         plus the algorithmic bytes of one full-array pass."""
-        ms = (ctypes.c_double * 7)()
+        ms = (ctypes.c_double * len(self.KERNELS))()
         tb = ctypes.c_uint64(0)
         _check(lib().pl_kernel_times(self.h, reps, ms, ctypes.byref(tb)))
         return dict(zip(self.KERNELS, list(ms))), tb.value

@@ -190,7 +190,7 @@ int pl_ctx_info(const pl_ctx* ctx, pl_info* out)
     out->cols = ctx->plan.C;
     out->bits_per_digit = (double)ctx->plan.n / (double)ctx->plan.L;
     out->device_bytes = ctx->eng->device_bytes();
-    out->passes = ctx->plan.nt ? 3 : 5;
+    out->passes = (uint32_t)ctx->eng->launches_per_squaring();
     return PL_OK;
 }
 
@@ -227,6 +227,7 @@ int pl_square(pl_ctx* ctx, uint64_t iters, double err_guard, double* max_err, do
         if (max_err) *max_err = e;
         int st = ctx->eng->status();
         if (st) ctx->eng->clear_status();
+        if (st & 2) return fail(PL_ERR_STATE, "carry hand-off between workgroups timed out");
         if (st) return fail(PL_ERR_STATE, "device carry did not settle (status " + std::to_string(st) + ")");
         if (e > err_guard)
             return fail(PL_ERR_ROUNDOFF, "round-off " + std::to_string(e) + " above guard " + std::to_string(err_guard));

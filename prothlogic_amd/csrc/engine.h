@@ -104,6 +104,12 @@ class Group {
 
     int status();
     void clear_status();
+    // kernels per steady-state squaring: 2 (one-launch outer pass + row
+    // pass), 3 (split outer pass in two launches + row pass) or 5 (unfused)
+    int launches_per_squaring() const { return turn() ? 2 : (plan_.nt ? 3 : 5); }
+    // the one-launch outer pass is used (runs of 8 digits; with 16 its
+    // registers would spill)
+    bool turn() const { return turn_ && plan_.nt == 8; }
 
     // Average duration (ms) of each kernel over all slots, timed with HIP
     // events between consecutive kernels of `reps` real plain squarings on
@@ -116,7 +122,7 @@ class Group {
     // entry/exit pass around the loop).  Slot 0's residue is saved and
     // restored; other slots are squared `reps` times.
     void kernel_times(int reps, double* ms);
-    static constexpr int kNumKernels = 7;
+    static constexpr int kNumKernels = 8;
     static constexpr int kChunk = 64;
 
   private:
@@ -156,6 +162,7 @@ class Group {
     std::map<uint64_t, std::pair<hipGraphExec_t, hipGraph_t>> plain_graphs_;
     uint32_t sq0_ = 0, sqn_ = 1;
     bool uniform_ = true;  // every slot holds the plan's n and k
+    bool turn_ = true;     // split outer pass as one launch (k_carry_turn); PL_CARRY_TURN=0: two
     std::vector<unsigned long long> err_host_;
 };
 

@@ -49,6 +49,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -81,7 +82,8 @@ struct Params {
     const double2* wci;
     const double2* tcq;     // W_L^(c*q*R/16), c < C, q < 16 (index c*16 + q)
     long long* carries;     // flat carry kernel: 3 per block (own carry-outs of its last run)
-    long long* runc;        // split outer pass: 3 carry-outs per run (block-major, then row)
+    long long* runc;        // split outer pass: 3 carry-outs per run (block-major, then row);
+                            // the one-launch pass stores them as tagged words
     int* dig;               // split outer pass: NT int32 digits per run (same order)
     long long* xhi;
     const long long* xhi_mul;  // product pass: the other factor's X_hi (nullptr: squaring)
@@ -371,27 +373,25 @@ __global__ __launch_bounds__(TPB) void k_inv_col(Params pb)
         [&](int tr, int idx, double2 v, int, int) { p.z[(size_t)idx * p.C + c0 + tr] = v; });
 }
 
-// Row pass: NT rows; forward C-point DFT, Z <- Z*(Z + 2i*X_hi) folded into
-// the last forward stage, inverse C-point DFT.  The stage twiddles are read
-// once and serve both directions.
+// Row pass: NT rows; forward C-point DFT, Z <- Z*(Z + 2i*X_hi), inverse
+// C-point DFT.  Where the first and last radix agree (C = 64, 512, 4096 at
+// radix 8) the forward last stage, the pointwise step and the inverse first
+// stage run in registers.  The stage twiddles are read once and serve both
+// directions.
 template <int C, int NT, int TPB>
 __global__ __launch_bounds__(TPB) void k_row(Params pb)
 {
     const Params p = bind(pb, blockIdx.y);
     if (!p.active) return;
-    using Lay = LdsLayout<C, NT>;
-    __shared__ double2 lds[Lay::kSlots];
+    __shared__ double2 lds[LdsLayout<C, NT>::kSlots];
     StageTw<C, NT, TPB, PL_ROW_RADIX> t;
     load_stage_tw(t, p.tw);
     const size_t base = (size_t)blockIdx.x * NT * C;
     const double2* __restrict__ z = p.z;
     const double x2 = 2.0 * (double)(*p.xhi);
-    dft<C, NT, TPB, false, false, true>(
+    dft_square_pass<C, NT, TPB, PL_ROW_RADIX>(
         lds, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
-        [&](int tr, int idx, double2 X, int, int) { lds[Lay::at(tr, idx)] = cmul(X, make_double2(X.x, X.y + x2)); });
-    __syncthreads();
-    dft<C, NT, TPB, true, true>(
-        lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; },
+        [&](double2 X) { return cmul(X, make_double2(X.x, X.y + x2)); },
         [&](int tr, int idx, double2 v, int, int) { p.z[base + (size_t)tr * C + idx] = v; });
 }
 
@@ -728,6 +728,195 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
         });
 }
 
+// The outer pass in one launch: inverse column DFT, unweight, round, own
+// carries of every run (as k_inv_carry), then each run's carry-outs are
+// handed straight to its successor run in the next column block, and the
+// block goes on with the carry-in, weighting and forward column DFT (as
+// k_adjust_fwd) without its digits leaving registers.
+//
+// Hand-off.  A run's three carry-outs are exact integers below 2^53 in
+// magnitude (they come out of FP64 arithmetic that is exact), so each goes
+// out as one 64-bit word q*2^10 + tag, where tag is the launch's epoch mod
+// 2^10, with an agent-scope (device-coherent) atomic store.  The successor's
+// thread polls the three words with agent-scope loads until all carry its
+// epoch: the data is its own ready flag, each word is written whole, and
+// nothing but these atomics crosses workgroups, so there is no L2 write-back
+// or invalidate (an agent-scope release/acquire pair made the launch four
+// times slower at L = 2^20, flushing the z lines other blocks had just
+// written) and no second round trip for a separate flag.  A run's epoch is
+// the tag its own words carry from the previous launch, plus one; all runs
+// of a slot go through the same launches, so they agree on it.  Every block
+// publishes before it waits, a block waits only on blockIdx.x - 1 (block 0
+// on the slot's last block), and workgroups start in increasing order on
+// each XCD, so the waits cannot form a cycle; a wait still gives up after
+// kSpinLimit polls (status bit 2, well over 0.1 s where a normal wait is
+// microseconds) so that no launch can hang.
+constexpr unsigned kSpinLimit = 1u << 18;
+constexpr int kTagBits = 10;
+constexpr unsigned kTagMask = (1u << kTagBits) - 1;
+
+__device__ __forceinline__ unsigned long long tag_word(double q, unsigned ep)
+{
+    return ((unsigned long long)(long long)q << kTagBits) | ep;
+}
+__device__ __forceinline__ long long untag(unsigned long long w) { return (long long)w >> kTagBits; }
+__device__ __forceinline__ unsigned tag_of(unsigned long long w) { return (unsigned)w & kTagMask; }
+
+template <int R, int NT, int TPB>
+__global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
+{
+    const Params p = bind(pb, blockIdx.y);
+    if (!p.active) return;
+    using Lay = LdsLayout<R, NT>;
+    constexpr int RPT = (R + TPB - 1) / TPB;
+    constexpr int P0 = Radix<R>::first();
+    constexpr int PL = Radix<R>::last();
+    __shared__ double2 lds[Lay::kSlots];
+    __shared__ double2 s_wc[2][NT];  // the block's column weight factors: inverse, forward
+    __shared__ double s_err[TPB / 64];
+    const unsigned nb = p.C / NT;
+    const unsigned b = blockIdx.x;
+    const unsigned c0 = b * NT;
+    const double2* __restrict__ z = p.z;
+    unsigned long long* const rw = reinterpret_cast<unsigned long long*>(p.runc);
+    StageTw<R, NT, TPB> t;
+    load_stage_tw(t, p.tw);
+    if (threadIdx.x < 2 * NT) {
+        const unsigned j = threadIdx.x % NT;
+        s_wc[threadIdx.x / NT][j] = threadIdx.x < NT ? p.wci[c0 + j] : p.wcf[c0 + j];
+    }
+    unsigned ep[RPT];
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        ep[u] = r < (unsigned)R
+                    ? (tag_of(__hip_atomic_load(rw + 3 * ((size_t)b * R + r), __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)) + 1u) & kTagMask
+                    : 0u;
+    }
+    double2 wri[RPT];
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        wri[u] = r < (unsigned)R ? p.wri[r] : make_double2(0.0, 0.0);
+    }
+    {
+        ColTw<R, NT, TPB, P0> ct;
+        ct.load(p, c0 + threadIdx.x % NT);
+        dft<R, NT, TPB, true, false, true>(
+            lds, t, [&](int tr, int idx, int u, int r) { return cmulc(z[(size_t)idx * p.C + c0 + tr], ct.w[u][r]); },
+            [&](int tr, int idx, double2 v, int, int) { lds[Lay::at(tr, idx)] = v; });
+    }
+    __syncthreads();
+
+    const unsigned long long Lm = p.L - 1;
+    const unsigned nr = (unsigned)(p.n & Lm);
+    unsigned G[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) G[j] = (unsigned)((0ull - p.n * (unsigned long long)(c0 + j)) & Lm);
+    double emax = 0.0;
+    const long long k = p.k;
+    const double kd = (double)p.k, kmd = (double)(p.k * p.mult);
+    const double tb0 = ldexp(1.0, (int)(p.n >> p.logL)), tb1 = 2.0 * tb0;
+    const double itb0 = 1.0 / tb0, itb1 = 0.5 * itb0;
+    int dg[RPT][NT];
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        if (r >= (unsigned)R) break;
+        const unsigned F = (unsigned)((0ull - p.n * (unsigned long long)r * p.C) & Lm);
+        double q1 = 0, q2 = 0, q3 = 0;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            double2 w = cmul(wri[u], s_wc[0][j]);
+            unsigned f = F + G[j];
+            const bool wrap = f >= p.L;
+            if (wrap) {
+                f -= p.L;
+                w = make_double2(2.0 * w.x, 2.0 * w.y);
+            }
+            const double2 v = cmul(lds[Lay::at(j, r)], w);
+            const double tr = rint(v.x), hr = rint(v.y);
+            emax = fmax(emax, fmax(fabs(v.x - tr), fabs(v.y - hr)));
+            const bool wd = f < nr;
+            dg[u][j] = (int)carry3f(tr, hr, kd, kmd, wd ? tb1 : tb0, wd ? itb1 : itb0, q1, q2, q3);
+        }
+        unsigned long long* w = rw + 3 * ((size_t)b * R + r);
+        __hip_atomic_store(w + 0, tag_word(q1, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(w + 1, tag_word(q2, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(w + 2, tag_word(q3, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    emax = wave_max(emax);
+    if ((threadIdx.x & 63) == 0) s_err[threadIdx.x / 64] = emax;
+
+    // forward-side tables, requested while the predecessors finish
+    ColTw<R, NT, TPB, PL> ct;
+    ct.load(p, c0 + threadIdx.x % NT);
+    double2 wrf[RPT];
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        wrf[u] = r < (unsigned)R ? p.wrf[r] : make_double2(0.0, 0.0);
+    }
+
+    // Predecessor of run (b, r): (b-1, r); (nb-1, r-1) for b = 0; run (0, 0)
+    // takes the top run (nb-1, R-1) through the top rule.  Each thread
+    // rewrites only the LDS slots of its own runs, which only it read in the
+    // carry phase: no barrier before the weighted digits go in.
+    bool late = false;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        if (r >= (unsigned)R) break;
+        const unsigned long long* pw =
+            rw + 3 * (b == 0 ? (r == 0 ? (size_t)(nb - 1) * R + (R - 1) : (size_t)(nb - 1) * R + (r - 1))
+                             : (size_t)(b - 1) * R + r);
+        unsigned long long w0, w1, w2;
+        unsigned spins = 0;
+        for (;;) {
+            w0 = __hip_atomic_load(pw + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            w1 = __hip_atomic_load(pw + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            w2 = __hip_atomic_load(pw + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tag_of(w0) == ep[u] && tag_of(w1) == ep[u] && tag_of(w2) == ep[u]) break;
+            if (++spins >= kSpinLimit) {
+                late = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const long long cin[3] = { untag(w0), untag(w1), untag(w2) };
+        __int128 c2in, extra;
+        if (b == 0 && r == 0) {
+            top_rule(p, cin, c2in, extra);
+            *p.xhi = cin[2];  // the old X_hi was read just above, in this thread
+        } else {
+            c2in = (__int128)cin[1] - (__int128)k * cin[0];
+            extra = cin[2];
+        }
+        long long nu[NT];
+        ripple<NT>(p, r * p.C + c0, c2in, extra, nu);
+        const unsigned F = (unsigned)((0ull - p.n * (unsigned long long)r * p.C) & Lm);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            double2 w = cmul(wrf[u], s_wc[1][j]);
+            if (F + G[j] >= p.L) w = make_double2(0.5 * w.x, 0.5 * w.y);
+            const double xd = (double)dg[u][j] + (double)nu[j];
+            lds[Lay::at(j, r)] = make_double2(xd * w.x, xd * w.y);
+        }
+    }
+    if (late) atomicOr(p.status, 2);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double e = s_err[0];
+#pragma unroll
+        for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
+        p.err[(size_t)blockIdx.x * Group::kChunk] = (unsigned long long)__double_as_longlong(e);
+    }
+    dft<R, NT, TPB, false, true, false>(
+        lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; },
+        [&](int tr, int idx, double2 v, int u, int r) { p.z[(size_t)idx * p.C + c0 + tr] = cmul(v, ct.w[u][r]); });
+}
+
 __global__ void k_set_digits(Params pb, const int* __restrict__ d)
 {
     const Params p = bind(pb, blockIdx.y);
@@ -929,7 +1118,17 @@ struct RowCfg {
     static constexpr int TPB = tpb_for(NT * C, PL_ROW_RADIX);
 };
 
-enum Kind { kFwdCol = 0, kRow = 1, kInvCol = 2, kCarry = 3, kFixup = 4, kInvCarry = 5, kAdjFwd = 6, kAdvance = 7 };
+enum Kind {
+    kFwdCol = 0,
+    kRow = 1,
+    kInvCol = 2,
+    kCarry = 3,
+    kFixup = 4,
+    kInvCarry = 5,
+    kAdjFwd = 6,
+    kCarryTurn = 7,
+    kAdvance = 8
+};
 
 template <int R>
 static void launch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, hipStream_t st)
@@ -946,6 +1145,7 @@ static void launch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, h
             constexpr int TPB = SplitCfg<R, 8>::TPB;
             dim3 grid(p.C / 8, slots);
             if (kind == kInvCarry) hipLaunchKernelGGL((k_inv_carry<R, 8, TPB>), grid, dim3(TPB), 0, st, p);
+            else if (kind == kCarryTurn) hipLaunchKernelGGL((k_carry_turn<R, 8, TPB>), grid, dim3(TPB), 0, st, p);
             else hipLaunchKernelGGL((k_adjust_fwd<R, 8, TPB>), grid, dim3(TPB), 0, st, p);
             return;
         }
@@ -953,7 +1153,8 @@ static void launch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, h
             constexpr int TPB = SplitCfg<R, 16>::TPB;
             dim3 grid(p.C / 16, slots);
             if (kind == kInvCarry) hipLaunchKernelGGL((k_inv_carry<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
-            else hipLaunchKernelGGL((k_adjust_fwd<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
+            else if (kind == kAdjFwd) hipLaunchKernelGGL((k_adjust_fwd<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
+            else throw std::runtime_error("the one-launch outer pass needs runs of 8 digits");
             return;
         }
     }
@@ -1123,6 +1324,7 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     host_tables(p, tw, twl_lo, twl_hi, om_lo, om_hi, pw_lo, pw_hi, ipw_lo, ipw_hi);
     carry_tpb_ = carry_tpb_for(p);
     carry_blocks_ = p.L / (p.run * carry_tpb_);
+    if (const char* e = std::getenv("PL_CARRY_TURN")) turn_ = e[0] != '0';
     const size_t runs = p.nt ? (size_t)p.L / p.nt : 0;
 
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -1381,7 +1583,9 @@ void Group::enqueue_plain_chunk(int g)
     launch(kRow, params(s0, false, 0), ns);
     for (int i = 0; i + 1 < g; ++i) {
         const Params p = params(s0, false, (uint32_t)i);
-        if (plan_.nt) {
+        if (turn()) {
+            launch(kCarryTurn, p, ns);
+        } else if (plan_.nt) {
             launch(kInvCarry, p, ns);
             launch(kAdjFwd, p, ns);
         } else {
@@ -1403,7 +1607,9 @@ void Group::enqueue_step_chunk(int g)
 {
     for (int i = 0; i < g; ++i) {
         const Params p = params(0, true, (uint32_t)i);
-        if (plan_.nt) {
+        if (turn()) {
+            launch(kCarryTurn, p, cap_);
+        } else if (plan_.nt) {
             launch(kInvCarry, p, cap_);
             launch(kAdjFwd, p, cap_);
         } else {
@@ -1623,7 +1829,9 @@ void Group::kernel_times(int reps, double* ms)
     // chunk (forward column pass from digits, inverse column + carry + fix-up
     // back to digits) are timed on the same pass around the loop.
     std::vector<int> seq;
-    if (plan_.nt)
+    if (turn())
+        seq = { kCarryTurn, kRow };
+    else if (plan_.nt)
         seq = { kInvCarry, kAdjFwd, kRow };
     else
         seq = { kInvCol, kCarry, kFixup, kFwdCol, kRow };

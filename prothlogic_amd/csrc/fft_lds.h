@@ -342,4 +342,111 @@ __device__ __forceinline__ void dft(double2* lds, const StageTw<N, NT, TPB, MAXR
     dft_rest<N, NT, TPB, MAXR, INV, IN_LDS, OUT_LDS, 1, 0>(lds, t, ld_in, st_out);
 }
 
+// Span and stage number of the last stage of an N-point transform.
+template <int N, int MAXR>
+struct LastStage {
+    static constexpr int ns()
+    {
+        int ns = 1, p = Radix<N, MAXR>::of(1);
+        while (ns * p < N) {
+            ns *= p;
+            p = Radix<N, MAXR>::of(ns);
+        }
+        return ns;
+    }
+    static constexpr int si()
+    {
+        int ns = 1, s = 0, p = Radix<N, MAXR>::of(1);
+        while (ns * p < N) {
+            ns *= p;
+            p = Radix<N, MAXR>::of(ns);
+            ++s;
+        }
+        return s;
+    }
+};
+
+// Every stage of a transform but the last: the first reads ld_in, the
+// others run in LDS; returns after a barrier with the data in LDS.
+template <int N, int NT, int TPB, int MAXR, bool INV, int NS, int SI, class LdIn>
+__device__ __forceinline__ void dft_head(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in)
+{
+    using Lay = LdsLayout<N, NT>;
+    constexpr int P = Radix<N, MAXR>::of(NS);
+    if constexpr (NS * P < N) {
+        auto lds_ld = [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; };
+        auto lds_st = [&](int tr, int idx, double2 v, int, int) { lds[Lay::at(tr, idx)] = v; };
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, lds_st);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, true>(t, lds_ld, lds_st);
+        __syncthreads();
+        dft_head<N, NT, TPB, MAXR, INV, NS * P, SI + 1>(lds, t, ld_in);
+    }
+}
+
+// The turn of a squaring pass, in registers: the forward transform's last
+// stage, a pointwise map of every spectrum value, and the inverse
+// transform's first stage.  Forward last-stage butterfly j produces the
+// values at j + r*N/P, exactly the inputs of inverse first-stage butterfly
+// j, so the turn costs one LDS round trip instead of two.  Needs the first
+// and last radix equal.  Reads and writes lds; ends without a barrier.
+template <int N, int NT, int TPB, int MAXR, class Pw>
+__device__ __forceinline__ void dft_turn(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, Pw pw)
+{
+    using Lay = LdsLayout<N, NT>;
+    constexpr int NS = LastStage<N, MAXR>::ns(), SI = LastStage<N, MAXR>::si();
+    constexpr int P = Radix<N, MAXR>::of(NS);
+    static_assert(P == Radix<N, MAXR>::first(), "turn needs equal first and last radix");
+    static_assert(NS > 1, "turn needs at least two stages");
+    using S = StageShape<N, NT, TPB, P>;
+    const int tr = threadIdx.x % NT;
+    double2 v[S::PER][P];
+#pragma unroll
+    for (int u = 0; u < S::PER; ++u) {
+        if (S::live(u)) {
+            const int j = S::j(u);
+#pragma unroll
+            for (int r = 0; r < P; ++r) v[u][r] = lds[Lay::at(tr, j + r * S::NBT)];
+            double2 w[P];
+            twiddles<P>(t.w[SI][u], w);
+#pragma unroll
+            for (int r = 1; r < P; ++r) v[u][r] = cmul(v[u][r], w[r]);
+            dftp<P, false>(v[u]);
+#pragma unroll
+            for (int r = 0; r < P; ++r) v[u][r] = pw(v[u][r]);
+            dftp<P, true>(v[u]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < S::PER; ++u) {
+        if (S::live(u)) {
+            const int j = S::j(u);
+#pragma unroll
+            for (int r = 0; r < P; ++r) lds[Lay::at(tr, j * P + r)] = v[u][r];
+        }
+    }
+}
+
+// Forward transform from ld_in, pointwise map, inverse transform to st_out,
+// with the turn in registers when the first and last radix agree.
+template <int N, int NT, int TPB, int MAXR, class LdIn, class Pw, class StOut>
+__device__ __forceinline__ void dft_square_pass(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in, Pw pw,
+                                                StOut st_out)
+{
+    using Lay = LdsLayout<N, NT>;
+    constexpr int P0 = Radix<N, MAXR>::first();
+    if constexpr (P0 == Radix<N, MAXR>::last() && LastStage<N, MAXR>::ns() > 1) {
+        dft_head<N, NT, TPB, MAXR, false, 1, 0>(lds, t, ld_in);
+        dft_turn<N, NT, TPB, MAXR>(lds, t, pw);
+        __syncthreads();
+        dft_rest<N, NT, TPB, MAXR, true, true, false, P0, 1>(lds, t, ld_in, st_out);
+    } else {
+        dft<N, NT, TPB, false, false, true>(
+            lds, t, ld_in, [&](int tr, int idx, double2 X, int, int) { lds[Lay::at(tr, idx)] = pw(X); });
+        __syncthreads();
+        dft<N, NT, TPB, true, true>(
+            lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; }, st_out);
+    }
+}
+
 }  // namespace pleng
