@@ -1077,9 +1077,15 @@ bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::strin
     out.L = L;
     out.logL = logL;
     // Column DFTs of at most 256 points (more, smaller column blocks: C/8
-    // workgroups) unless the row DFT would pass 4096 points.  At n = 2^24
-    // (L = 2^20) 256 x 4096 squares in 36 us against 42 us for 512 x 2048.
-    const uint32_t logR = std::max<uint32_t>(logL > 12 ? logL - 12 : 0, std::min<uint32_t>(8, logL / 2));
+    // workgroups) unless the row DFT would pass 4096 points, and of R =
+    // 2^(logL/2 - 1) for L = 2^14..2^16.  Measured on MI355X (us per squaring,
+    // one-launch outer pass): L = 2^14 64x256 10.9 vs 128x128 12.2; 2^15
+    // 64x512 10.7 vs 128x256 12.1; 2^16 128x512 12.3 vs 256x256 13.7; 2^17
+    // 256x512 14.5 vs 128x1024 14.7; 2^19 256x2048 24.5 vs 128x4096 27.0 and
+    // 512x1024 28.0; 2^20 256x4096 30.9 vs 512x2048 37.5.
+    const uint32_t logR = logL >= 14 && logL <= 16
+                              ? logL / 2 - 1
+                              : std::max<uint32_t>(logL > 12 ? logL - 12 : 0, std::min<uint32_t>(8, logL / 2));
     out.R = 1u << logR;
     out.C = L / out.R;
     out.s = (logL + 1) / 2;
@@ -1112,9 +1118,12 @@ struct SplitCfg {
     static constexpr int TPB = tpb_for(NT * R);
 };
 
+// Row blocks of 512 points (NT = 512 / C rows for short rows): more, smaller
+// workgroups than 1024-point blocks, 0.6-0.9 us per squaring faster at
+// C <= 512 on MI355X and the same for longer rows.
 template <int C>
 struct RowCfg {
-    static constexpr int NT = (1024 / C) > 1 ? (1024 / C) : 1;
+    static constexpr int NT = (512 / C) > 1 ? (512 / C) : 1;
     static constexpr int TPB = tpb_for(NT * C, PL_ROW_RADIX);
 };
 
