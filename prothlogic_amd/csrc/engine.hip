@@ -750,7 +750,10 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
 // on the slot's last block), and workgroups start in increasing order on
 // each XCD, so the waits cannot form a cycle; a wait still gives up after
 // kSpinLimit polls (status bit 2, well over 0.1 s where a normal wait is
-// microseconds) so that no launch can hang.
+// microseconds) so that no launch can hang.  The words lie in three planes
+// (word i of run q at runc[i*runs + q]), so the 64 lanes of a wave store and
+// poll 512 contiguous bytes per instruction instead of touching each line
+// three times.
 constexpr unsigned kSpinLimit = 1u << 18;
 constexpr int kTagBits = 10;
 constexpr unsigned kTagMask = (1u << kTagBits) - 1;
@@ -779,6 +782,7 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     const unsigned c0 = b * NT;
     const double2* __restrict__ z = p.z;
     unsigned long long* const rw = reinterpret_cast<unsigned long long*>(p.runc);
+    const size_t runs = (size_t)nb * R;
     StageTw<R, NT, TPB> t;
     load_stage_tw(t, p.tw);
     if (threadIdx.x < 2 * NT) {
@@ -790,7 +794,7 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     for (int u = 0; u < RPT; ++u) {
         const unsigned r = threadIdx.x + u * TPB;
         ep[u] = r < (unsigned)R
-                    ? (tag_of(__hip_atomic_load(rw + 3 * ((size_t)b * R + r), __ATOMIC_RELAXED,
+                    ? (tag_of(__hip_atomic_load(rw + (size_t)b * R + r, __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT)) + 1u) & kTagMask
                     : 0u;
     }
@@ -841,10 +845,10 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
             const bool wd = f < nr;
             dg[u][j] = (int)carry3f(tr, hr, kd, kmd, wd ? tb1 : tb0, wd ? itb1 : itb0, q1, q2, q3);
         }
-        unsigned long long* w = rw + 3 * ((size_t)b * R + r);
-        __hip_atomic_store(w + 0, tag_word(q1, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(w + 1, tag_word(q2, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(w + 2, tag_word(q3, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long* w = rw + (size_t)b * R + r;
+        __hip_atomic_store(w, tag_word(q1, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(w + runs, tag_word(q2, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(w + 2 * runs, tag_word(q3, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     emax = wave_max(emax);
     if ((threadIdx.x & 63) == 0) s_err[threadIdx.x / 64] = emax;
@@ -869,14 +873,14 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         const unsigned r = threadIdx.x + u * TPB;
         if (r >= (unsigned)R) break;
         const unsigned long long* pw =
-            rw + 3 * (b == 0 ? (r == 0 ? (size_t)(nb - 1) * R + (R - 1) : (size_t)(nb - 1) * R + (r - 1))
-                             : (size_t)(b - 1) * R + r);
+            rw + (b == 0 ? (r == 0 ? (size_t)(nb - 1) * R + (R - 1) : (size_t)(nb - 1) * R + (r - 1))
+                         : (size_t)(b - 1) * R + r);
         unsigned long long w0, w1, w2;
         unsigned spins = 0;
         for (;;) {
-            w0 = __hip_atomic_load(pw + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            w1 = __hip_atomic_load(pw + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            w2 = __hip_atomic_load(pw + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            w0 = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            w1 = __hip_atomic_load(pw + runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            w2 = __hip_atomic_load(pw + 2 * runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (tag_of(w0) == ep[u] && tag_of(w1) == ep[u] && tag_of(w2) == ep[u]) break;
             if (++spins >= kSpinLimit) {
                 late = true;
