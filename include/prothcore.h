@@ -71,8 +71,9 @@ int pl_autotune_fft_len(uint32_t n, uint64_t k, int device, double err_guard, ui
 
 /* Context create/destroy (replaces fft_init / fft_destroy,
  * P/fft_engine.h:10-11).  fft_len = 0 picks pl_plan_fft_len's length.
- * The residue starts at 0.  The environment variable PL_FUSED=0 disables
- * the split outer pass (5 kernels per squaring instead of 3). */
+ * The residue starts at 0.  Environment (read at creation): PL_CARRY_TURN=0
+ * runs the outer pass as two launches (3 kernels per squaring instead of 2);
+ * PL_FUSED=0 runs it unfused (5 kernels).  pl_info.passes reports which. */
 int pl_ctx_create(uint64_t k, uint32_t n, uint32_t fft_len, int device, pl_ctx** out);
 void pl_ctx_destroy(pl_ctx* ctx);
 int pl_ctx_info(const pl_ctx* ctx, pl_info* out);
