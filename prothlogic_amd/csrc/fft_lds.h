@@ -23,10 +23,11 @@
 // (natural order out).  The first stage therefore reads idx = j + r*N/P0 and
 // the last writes idx = j + r*N/P_last.
 //
-// Stage twiddles are loaded once per kernel into registers (StageTw) before
-// the data arrive: per stage and butterfly the powers W^e, W^2e, W^4e, W^8e
-// come from the 4096-point table and the other powers are their products,
-// formed at use; the inverse transform uses the same registers conjugated.
+// Stage twiddles are formed once per kernel in registers (StageTw) before
+// the data arrive: per stage and butterfly W^e comes from the 4096-point
+// table, W^2e, W^4e, W^8e are its squares and the other powers their
+// products, formed at use; the inverse transform uses the same registers
+// conjugated.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -216,7 +217,7 @@ struct StageShape {
 // Register-resident stage twiddles of one thread for every stage of an
 // N-point transform: for the stage with span NS and butterfly u, w[stage][u]
 // holds W^e, W^2e, W^4e, W^8e with W^e = W_(NS*P)^(j mod NS); the other
-// powers are formed where they are used (twiddle()).
+// powers are formed where they are used (twiddles()).
 template <int N, int NT, int TPB, int MAXR = PL_MAX_RADIX>
 struct StageTw {
     static constexpr int G = TPB / NT;
@@ -257,13 +258,16 @@ __device__ __forceinline__ void tw_fill(StageTw<N, NT, TPB, MAXR>& t, const doub
 #pragma unroll
         for (int u = 0; u < S::PER; ++u) {
             if (!S::live(u)) continue;
-            // e < (N/P) * SCALE = 4096/P, so the largest index read,
-            // (P/2) * e, stays below 2048.
+            // e < (N/P) * SCALE = 4096/P: the index stays in the table.
+            // W^e from the table, W^2e, W^4e, W^8e by squaring: one load per
+            // stage and butterfly instead of up to four (each load a thread
+            // issues before its data costs ~0.2 us at L = 2^20; the squarings
+            // add a few ulp to a twiddle, far inside the round-off guard)
             const int e = (S::j(u) % NS) * STEP * SCALE;
             t.w[SI][u][0] = tw[e];
-            if constexpr (P >= 4) t.w[SI][u][1] = tw[2 * e];
-            if constexpr (P >= 8) t.w[SI][u][2] = tw[4 * e];
-            if constexpr (P >= 16) t.w[SI][u][3] = tw[8 * e];
+            if constexpr (P >= 4) t.w[SI][u][1] = cmul(t.w[SI][u][0], t.w[SI][u][0]);
+            if constexpr (P >= 8) t.w[SI][u][2] = cmul(t.w[SI][u][1], t.w[SI][u][1]);
+            if constexpr (P >= 16) t.w[SI][u][3] = cmul(t.w[SI][u][2], t.w[SI][u][2]);
         }
     }
     if constexpr (NS * P < N) tw_fill<N, NT, TPB, MAXR, NS * P, SI + 1>(t, tw);
