@@ -163,6 +163,7 @@ class Group {
     uint32_t sq0_ = 0, sqn_ = 1;
     bool uniform_ = true;  // every slot holds the plan's n and k
     bool turn_ = true;     // split outer pass as one launch (k_carry_turn); PL_CARRY_TURN=0: two
+    bool give_up_ = false; // PL_HANDOFF_GIVE_UP=1: every carry hand-off wait gives up (tests)
     std::vector<unsigned long long> err_host_;
 };
 

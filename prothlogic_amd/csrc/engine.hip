@@ -96,6 +96,8 @@ struct Params {
     unsigned err_stride;
     unsigned slot0;         // first slot of this launch
     int sched;              // apply per-slot multipliers and the active flag
+    unsigned spin_limit;    // polls before a carry hand-off wait gives up (status bit 2)
+    unsigned tag_skew;      // 0; a test of the give-up path expects tags that never come
     int per_slot;           // slots hold different candidates: read n, k from desc
                             // (else n, k below are every slot's, and no kernel
                             // starts with a dependent descriptor load)
@@ -757,7 +759,9 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
 // on the slot's last block), and workgroups start in increasing order on
 // each XCD, so the waits cannot form a cycle; a wait still gives up after
 // kSpinLimit polls (status bit 2, well over 0.1 s where a normal wait is
-// microseconds) so that no launch can hang.  The words lie in three planes
+// microseconds) so that no launch can hang.  PL_HANDOFF_GIVE_UP=1 (tests of
+// that path) makes every wait expect a tag no run writes and give up after
+// 64 polls.  The words lie in three planes
 // (word i of run q at runc[i*runs + q]), so the 64 lanes of a wave store and
 // poll 512 contiguous bytes per instruction instead of touching each line
 // three times.
@@ -888,8 +892,9 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
             w0 = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             w1 = __hip_atomic_load(pw + runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             w2 = __hip_atomic_load(pw + 2 * runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (tag_of(w0) == ep[u] && tag_of(w1) == ep[u] && tag_of(w2) == ep[u]) break;
-            if (++spins >= kSpinLimit) {
+            const unsigned want = (ep[u] + p.tag_skew) & kTagMask;
+            if (tag_of(w0) == want && tag_of(w1) == want && tag_of(w2) == want) break;
+            if (++spins >= p.spin_limit) {
                 late = true;
                 break;
             }
@@ -1345,6 +1350,7 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     carry_tpb_ = carry_tpb_for(p);
     carry_blocks_ = p.L / (p.run * carry_tpb_);
     if (const char* e = std::getenv("PL_CARRY_TURN")) turn_ = e[0] != '0';
+    if (const char* e = std::getenv("PL_HANDOFF_GIVE_UP")) give_up_ = e[0] == '1';
     const size_t runs = p.nt ? (size_t)p.L / p.nt : 0;
 
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -1488,6 +1494,8 @@ Params Group::params(uint32_t slot0, bool check_active, uint32_t err_col) const
     p.err_stride = kChunk * eb_;
     p.slot0 = slot0;
     p.sched = check_active ? 1 : 0;
+    p.spin_limit = give_up_ ? 64u : kSpinLimit;
+    p.tag_skew = give_up_ ? 1u : 0u;
     p.per_slot = uniform_ ? 0 : 1;
     p.n = plan_.n;
     p.k = (long long)plan_.k;
