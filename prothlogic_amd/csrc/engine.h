@@ -123,7 +123,12 @@ class Group {
     // restored; other slots are squared `reps` times.
     void kernel_times(int reps, double* ms);
     static constexpr int kNumKernels = 8;
-    static constexpr int kChunk = 64;
+    // squarings per plain chunk (one graph replay, one round-off read-back)
+    // and columns of the round-off arrays; a chunk enters and leaves the
+    // steady state once, so longer chunks spread that cost thinner
+    static constexpr int kChunk = 256;
+    // steps per pool chunk (the pool retires and admits between chunks)
+    static constexpr int kStepChunk = 64;
 
   private:
     Params params(uint32_t slot0, bool check_active, uint32_t err_col) const;

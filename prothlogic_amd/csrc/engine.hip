@@ -1788,9 +1788,9 @@ double Group::step(uint32_t steps, double* slot_err)
     std::vector<double> se(cap_, 0.0), tmp(cap_, 0.0);
     uint32_t done = 0;
     while (done < steps) {
-        const int g = (int)std::min<uint32_t>(kChunk, steps - done);
-        if (g == kChunk) {
-            if (!step_exec_) build_graph(step_exec_, step_graph_, true, kChunk);
+        const int g = (int)std::min<uint32_t>(kStepChunk, steps - done);
+        if (g == kStepChunk) {
+            if (!step_exec_) build_graph(step_exec_, step_graph_, true, kStepChunk);
             check(hipGraphLaunch(step_exec_, stream_), "hipGraphLaunch");
         } else {
             enqueue_err_clear();

@@ -1,8 +1,8 @@
 """Short, fixed program for hardware-counter passes over the squaring kernels.
 
-    python -m prothlogic_amd.profile_run --workload c3 --iters 64 [--copy-mib 256]
+    python -m prothlogic_amd.profile_run --workload c3 --iters 256 [--copy-mib 256]
 
-Runs `iters` plain squarings (one graph-captured chunk per 64) of the named
+Runs `iters` plain squarings (one graph-captured chunk per 256) of the named
 single-test workload (c2: n=2^21, c3: n=2^24) from a random residue, then one
 plain device-to-device copy of `copy-mib` MiB (reads and writes exactly that
 many bytes, pl_copy_probe).  The copy calibrates the per-dispatch
@@ -24,7 +24,7 @@ sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=("c2", "c3"), default="c3")
-    ap.add_argument("--iters", type=int, default=64)
+    ap.add_argument("--iters", type=int, default=256)
     ap.add_argument("--copy-mib", type=int, default=256)
     args = ap.parse_args()
 

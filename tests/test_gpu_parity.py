@@ -53,7 +53,9 @@ def test_fused_and_unfused_paths(gpu, monkeypatch, mode, k, n, L):
     rnd = random.Random(n)
     N = k * (1 << n) + 1
     x = rnd.randrange(N)
-    iters = 130 if n < 10 ** 6 else 70      # > one 64-squaring graph chunk plus a partial one
+    # shorter lengths: one 256-squaring graph chunk plus a partial one; the
+    # long lengths' full chunks are covered by test_large_golden_c2_c3
+    iters = 300 if n < 10 ** 6 else 70
     print(f"[parity] mode={mode} k={k} n={n} L={L} iters={iters}", flush=True)
     with gpu.Context(k, n, L) as c:
         if mode == "unfused":
