@@ -192,13 +192,17 @@ def plain_per_squaring(core, k, n, iters=1500):
     return dt / iters, L
 
 
-def batch_c4(core, workloads, steps=1000):
+def batch_c4(core, workloads, steps=10000):
     """Config 4: 256 candidates, n uniform in [2^18, 2^20], advanced together
     by the pool (one launch per kernel kind per length group per squaring
     step).  Each test is run for its seed powering plus `steps` squarings;
     tests/hour extrapolates the measured squaring throughput to full tests
-    (topbit(k) + n - 1 squarings each).  The plain loop runs one candidate at
-    a time: its per-squaring time is measured per transform length."""
+    (topbit(k) + n - 1 squarings each).  The timed region includes creating
+    the length groups, capturing their graphs, admitting and retiring every
+    candidate: 10^4 steps keep that one-time cost near the share it has in a
+    full test (2.6*10^5..10^6 squarings); at 10^3 steps it halved the
+    measured throughput.  The plain loop runs one candidate at a time: its
+    per-squaring time is measured per transform length."""
     w = workloads.c4()
     cands = w.candidates
     full = [_topbit(k) + n - 1 for k, n in cands]
