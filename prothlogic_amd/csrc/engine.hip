@@ -385,14 +385,17 @@ __global__ __launch_bounds__(TPB) void k_row(Params pb)
 {
     const Params p = bind(pb, blockIdx.y);
     if (!p.active) return;
-    __shared__ double2 lds[LdsLayout<C, NT>::kSlots];
+    // two LDS buffers (ping-pong stages, half the barriers); at C = 4096
+    // that is 128 KB, which one workgroup per CU has to itself anyway
+    __shared__ double2 lds[2][LdsLayout<C, NT>::kSlots];
+    double2* const buf[2] = { lds[0], lds[1] };
     StageTw<C, NT, TPB, PL_ROW_RADIX> t;
     load_stage_tw(t, p.tw);
     const size_t base = (size_t)blockIdx.x * NT * C;
     const double2* __restrict__ z = p.z;
     const double x2 = 2.0 * (double)(*p.xhi);
-    dft_square_pass<C, NT, TPB, PL_ROW_RADIX>(
-        lds, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
+    dft_square_pass_pp<C, NT, TPB, PL_ROW_RADIX>(
+        buf, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
         [&](double2 X) { return cmul(X, make_double2(X.x, X.y + x2)); },
         [&](int tr, int idx, double2 v, int, int) { p.z[base + (size_t)tr * C + idx] = v; });
 }
@@ -785,7 +788,13 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     constexpr int RPT = (R + TPB - 1) / TPB;
     constexpr int P0 = Radix<R>::first();
     constexpr int PL = Radix<R>::last();
-    __shared__ double2 lds[Lay::kSlots];
+    // Two LDS buffers (ping-pong stages, half the barriers) while two
+    // workgroups still fit a CU; one buffer for R*NT > 2048 points.
+    constexpr bool PP = Lay::kSlots <= 2048;
+    constexpr int IO = pp_out<R, PL_MAX_RADIX, 0>();  // buffer the inverse DFT leaves the columns in
+    __shared__ double2 lds[PP ? 2 : 1][Lay::kSlots];
+    double2* const buf[2] = { lds[0], lds[PP ? 1 : 0] };
+    double2* const io = buf[PP ? IO : 0];
     __shared__ double2 s_wc[2][NT];  // the block's column weight factors: inverse, forward
     __shared__ double s_err[TPB / 64];
     const unsigned nb = p.C / NT;
@@ -818,9 +827,10 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     {
         ColTw<R, NT, TPB, P0> ct;
         ct.load(p, c0 + threadIdx.x % NT);
-        dft<R, NT, TPB, true, false, true>(
-            lds, t, [&](int tr, int idx, int u, int r) { return cmulc(z[(size_t)idx * p.C + c0 + tr], ct.w[u][r]); },
-            [&](int tr, int idx, double2 v, int, int) { lds[Lay::at(tr, idx)] = v; });
+        auto ld = [&](int tr, int idx, int u, int r) { return cmulc(z[(size_t)idx * p.C + c0 + tr], ct.w[u][r]); };
+        auto st = [&](int tr, int idx, double2 v, int, int) { io[Lay::at(tr, idx)] = v; };
+        if constexpr (PP) dft_pp<R, NT, TPB, true, 0>(buf, t, ld, st);
+        else dft<R, NT, TPB, true, false, true>(io, t, ld, st);
     }
     __syncthreads();
 
@@ -850,7 +860,7 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
                 f -= p.L;
                 w = make_double2(2.0 * w.x, 2.0 * w.y);
             }
-            const double2 v = cmul(lds[Lay::at(j, r)], w);
+            const double2 v = cmul(io[Lay::at(j, r)], w);
             const double tr = rint(v.x), hr = rint(v.y);
             emax = fmax(emax, fmax(fabs(v.x - tr), fabs(v.y - hr)));
             const bool wd = f < nr;
@@ -917,7 +927,7 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
             double2 w = cmul(wrf[u], s_wc[1][j]);
             if (F + G[j] >= p.L) w = make_double2(0.5 * w.x, 0.5 * w.y);
             const double xd = (double)dg[u][j] + (double)nu[j];
-            lds[Lay::at(j, r)] = make_double2(xd * w.x, xd * w.y);
+            io[Lay::at(j, r)] = make_double2(xd * w.x, xd * w.y);
         }
     }
     if (late) atomicOr(p.status, 2);
@@ -928,9 +938,10 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
         p.err[(size_t)blockIdx.x * Group::kChunk] = (unsigned long long)__double_as_longlong(e);
     }
-    dft<R, NT, TPB, false, true, false>(
-        lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; },
-        [&](int tr, int idx, double2 v, int u, int r) { p.z[(size_t)idx * p.C + c0 + tr] = cmul(v, ct.w[u][r]); });
+    auto ld = [&](int tr, int idx, int, int) { return io[Lay::at(tr, idx)]; };
+    auto st = [&](int tr, int idx, double2 v, int u, int r) { p.z[(size_t)idx * p.C + c0 + tr] = cmul(v, ct.w[u][r]); };
+    if constexpr (PP) dft_pp<R, NT, TPB, false, 1 - IO>(buf, t, ld, st);
+    else dft<R, NT, TPB, false, true, false>(io, t, ld, st);
 }
 
 __global__ void k_set_digits(Params pb, const int* __restrict__ d)

@@ -453,4 +453,133 @@ __device__ __forceinline__ void dft_square_pass(double2* lds, const StageTw<N, N
     }
 }
 
+// ------------------------------------------------------------ ping-pong
+//
+// The same transforms over two LDS buffers: a stage reads one buffer and
+// writes the other, so it needs no barrier between its reads and its writes,
+// only the one after its writes (the stage before last read the buffer this
+// stage writes, and that barrier orders them).  Half the barriers of the
+// one-buffer form, for twice the LDS.  A stage with span NS writes buf[DST]
+// (or st_out when it is the last) and reads buf[1 - DST] (ld_in when NS = 1).
+
+template <int N, int MAXR = PL_MAX_RADIX>
+struct Stages {
+    static constexpr int count()
+    {
+        int ns = 1, s = 1, p = Radix<N, MAXR>::of(1);
+        while (ns * p < N) {
+            ns *= p;
+            p = Radix<N, MAXR>::of(ns);
+            ++s;
+        }
+        return s;
+    }
+};
+
+template <int N, int NT, int TPB, int MAXR, bool INV, int NS, int SI, int DST, class LdIn, class StOut>
+__device__ __forceinline__ void dft_pp_rest(double2* const (&buf)[2], const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in,
+                                            StOut st_out)
+{
+    using Lay = LdsLayout<N, NT>;
+    constexpr int P = Radix<N, MAXR>::of(NS);
+    auto rd = [&](int tr, int idx, int, int) { return buf[1 - DST][Lay::at(tr, idx)]; };
+    auto wr = [&](int tr, int idx, double2 v, int, int) { buf[DST][Lay::at(tr, idx)] = v; };
+    if constexpr (NS * P >= N) {
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, st_out);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, rd, st_out);
+    } else {
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, wr);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, rd, wr);
+        __syncthreads();
+        dft_pp_rest<N, NT, TPB, MAXR, INV, NS * P, SI + 1, 1 - DST>(buf, t, ld_in, st_out);
+    }
+}
+
+// Full N-point DFT over two buffers; the first stage writes buf[DST0].  When
+// st_out writes LDS it must write buf[pp_out<N, MAXR, DST0>()], the buffer
+// the last stage does not read.  Ends without a trailing barrier.
+template <int N, int MAXR, int DST0>
+__device__ constexpr int pp_out()
+{
+    return (DST0 + Stages<N, MAXR>::count() - 1) & 1;
+}
+
+template <int N, int NT, int TPB, bool INV, int DST0, int MAXR, class LdIn, class StOut>
+__device__ __forceinline__ void dft_pp(double2* const (&buf)[2], const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in,
+                                       StOut st_out)
+{
+    dft_pp_rest<N, NT, TPB, MAXR, INV, 1, 0, DST0>(buf, t, ld_in, st_out);
+}
+
+// Every stage but the last, ping-pong; returns after a barrier with the data
+// in buf[pp_head_out<N, MAXR, DST0>()].
+template <int N, int MAXR, int DST0>
+__device__ constexpr int pp_head_out()
+{
+    return (DST0 + Stages<N, MAXR>::count() - 2) & 1;
+}
+
+template <int N, int NT, int TPB, int MAXR, bool INV, int NS, int SI, int DST, class LdIn>
+__device__ __forceinline__ void dft_pp_head(double2* const (&buf)[2], const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in)
+{
+    using Lay = LdsLayout<N, NT>;
+    constexpr int P = Radix<N, MAXR>::of(NS);
+    if constexpr (NS * P < N) {
+        auto rd = [&](int tr, int idx, int, int) { return buf[1 - DST][Lay::at(tr, idx)]; };
+        auto wr = [&](int tr, int idx, double2 v, int, int) { buf[DST][Lay::at(tr, idx)] = v; };
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, wr);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, rd, wr);
+        __syncthreads();
+        dft_pp_head<N, NT, TPB, MAXR, INV, NS * P, SI + 1, 1 - DST>(buf, t, ld_in);
+    }
+}
+
+// The squaring pass over two buffers: forward DFT from ld_in, pointwise map,
+// inverse DFT to st_out, with the turn in registers when the first and last
+// radix agree.
+template <int N, int NT, int TPB, int MAXR, class LdIn, class Pw, class StOut>
+__device__ __forceinline__ void dft_square_pass_pp(double2* const (&buf)[2], const StageTw<N, NT, TPB, MAXR>& t,
+                                                   LdIn ld_in, Pw pw, StOut st_out)
+{
+    using Lay = LdsLayout<N, NT>;
+    constexpr int P0 = Radix<N, MAXR>::first();
+    if constexpr (P0 == Radix<N, MAXR>::last() && LastStage<N, MAXR>::ns() > 1) {
+        constexpr int H = pp_head_out<N, MAXR, 0>();  // the forward head's output
+        dft_pp_head<N, NT, TPB, MAXR, false, 1, 0, 0>(buf, t, ld_in);
+        // the turn: forward last stage from buf[H], inverse first stage into buf[1 - H]
+        constexpr int NS = LastStage<N, MAXR>::ns(), SI = LastStage<N, MAXR>::si();
+        using S = StageShape<N, NT, TPB, P0>;
+        const int tr = threadIdx.x % NT;
+#pragma unroll
+        for (int u = 0; u < S::PER; ++u) {
+            if (S::live(u)) {
+                const int j = S::j(u);
+                double2 v[P0];
+#pragma unroll
+                for (int r = 0; r < P0; ++r) v[r] = buf[H][Lay::at(tr, j + r * S::NBT)];
+                double2 w[P0];
+                twiddles<P0>(t.w[SI][u], w);
+#pragma unroll
+                for (int r = 1; r < P0; ++r) v[r] = cmul(v[r], w[r]);
+                dftp<P0, false>(v);
+#pragma unroll
+                for (int r = 0; r < P0; ++r) v[r] = pw(v[r]);
+                dftp<P0, true>(v);
+#pragma unroll
+                for (int r = 0; r < P0; ++r) buf[1 - H][Lay::at(tr, j * P0 + r)] = v[r];
+            }
+        }
+        (void)NS;
+        __syncthreads();
+        dft_pp_rest<N, NT, TPB, MAXR, true, P0, 1, H>(buf, t, ld_in, st_out);
+    } else {
+        constexpr int F = pp_out<N, MAXR, 0>();  // where the forward pass leaves the spectrum
+        dft_pp<N, NT, TPB, false, 0>(buf, t, ld_in,
+                                     [&](int tr, int idx, double2 X, int, int) { buf[F][Lay::at(tr, idx)] = pw(X); });
+        __syncthreads();
+        dft_pp<N, NT, TPB, true, 1 - F>(buf, t, [&](int tr, int idx, int, int) { return buf[F][Lay::at(tr, idx)]; },
+                                        st_out);
+    }
+}
+
 }  // namespace pleng
