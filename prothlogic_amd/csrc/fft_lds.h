@@ -370,89 +370,6 @@ struct LastStage {
     }
 };
 
-// Every stage of a transform but the last: the first reads ld_in, the
-// others run in LDS; returns after a barrier with the data in LDS.
-template <int N, int NT, int TPB, int MAXR, bool INV, int NS, int SI, class LdIn>
-__device__ __forceinline__ void dft_head(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in)
-{
-    using Lay = LdsLayout<N, NT>;
-    constexpr int P = Radix<N, MAXR>::of(NS);
-    if constexpr (NS * P < N) {
-        auto lds_ld = [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; };
-        auto lds_st = [&](int tr, int idx, double2 v, int, int) { lds[Lay::at(tr, idx)] = v; };
-        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, lds_st);
-        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, true>(t, lds_ld, lds_st);
-        __syncthreads();
-        dft_head<N, NT, TPB, MAXR, INV, NS * P, SI + 1>(lds, t, ld_in);
-    }
-}
-
-// The turn of a squaring pass, in registers: the forward transform's last
-// stage, a pointwise map of every spectrum value, and the inverse
-// transform's first stage.  Forward last-stage butterfly j produces the
-// values at j + r*N/P, exactly the inputs of inverse first-stage butterfly
-// j, so the turn costs one LDS round trip instead of two.  Needs the first
-// and last radix equal.  Reads and writes lds; ends without a barrier.
-template <int N, int NT, int TPB, int MAXR, class Pw>
-__device__ __forceinline__ void dft_turn(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, Pw pw)
-{
-    using Lay = LdsLayout<N, NT>;
-    constexpr int NS = LastStage<N, MAXR>::ns(), SI = LastStage<N, MAXR>::si();
-    constexpr int P = Radix<N, MAXR>::of(NS);
-    static_assert(P == Radix<N, MAXR>::first(), "turn needs equal first and last radix");
-    static_assert(NS > 1, "turn needs at least two stages");
-    using S = StageShape<N, NT, TPB, P>;
-    const int tr = threadIdx.x % NT;
-    double2 v[S::PER][P];
-#pragma unroll
-    for (int u = 0; u < S::PER; ++u) {
-        if (S::live(u)) {
-            const int j = S::j(u);
-#pragma unroll
-            for (int r = 0; r < P; ++r) v[u][r] = lds[Lay::at(tr, j + r * S::NBT)];
-            double2 w[P];
-            twiddles<P>(t.w[SI][u], w);
-#pragma unroll
-            for (int r = 1; r < P; ++r) v[u][r] = cmul(v[u][r], w[r]);
-            dftp<P, false>(v[u]);
-#pragma unroll
-            for (int r = 0; r < P; ++r) v[u][r] = pw(v[u][r]);
-            dftp<P, true>(v[u]);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < S::PER; ++u) {
-        if (S::live(u)) {
-            const int j = S::j(u);
-#pragma unroll
-            for (int r = 0; r < P; ++r) lds[Lay::at(tr, j * P + r)] = v[u][r];
-        }
-    }
-}
-
-// Forward transform from ld_in, pointwise map, inverse transform to st_out,
-// with the turn in registers when the first and last radix agree.
-template <int N, int NT, int TPB, int MAXR, class LdIn, class Pw, class StOut>
-__device__ __forceinline__ void dft_square_pass(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in, Pw pw,
-                                                StOut st_out)
-{
-    using Lay = LdsLayout<N, NT>;
-    constexpr int P0 = Radix<N, MAXR>::first();
-    if constexpr (P0 == Radix<N, MAXR>::last() && LastStage<N, MAXR>::ns() > 1) {
-        dft_head<N, NT, TPB, MAXR, false, 1, 0>(lds, t, ld_in);
-        dft_turn<N, NT, TPB, MAXR>(lds, t, pw);
-        __syncthreads();
-        dft_rest<N, NT, TPB, MAXR, true, true, false, P0, 1>(lds, t, ld_in, st_out);
-    } else {
-        dft<N, NT, TPB, false, false, true>(
-            lds, t, ld_in, [&](int tr, int idx, double2 X, int, int) { lds[Lay::at(tr, idx)] = pw(X); });
-        __syncthreads();
-        dft<N, NT, TPB, true, true>(
-            lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; }, st_out);
-    }
-}
-
 // ------------------------------------------------------------ ping-pong
 //
 // The same transforms over two LDS buffers: a stage reads one buffer and
@@ -535,8 +452,11 @@ __device__ __forceinline__ void dft_pp_head(double2* const (&buf)[2], const Stag
 }
 
 // The squaring pass over two buffers: forward DFT from ld_in, pointwise map,
-// inverse DFT to st_out, with the turn in registers when the first and last
-// radix agree.
+// inverse DFT to st_out.  Where the first and last radix agree, the turn runs
+// in registers: forward last-stage butterfly j produces the values at
+// j + r*N/P, exactly the inputs of inverse first-stage butterfly j, so the
+// forward last stage, the pointwise map and the inverse first stage need one
+// LDS round trip instead of two.
 template <int N, int NT, int TPB, int MAXR, class LdIn, class Pw, class StOut>
 __device__ __forceinline__ void dft_square_pass_pp(double2* const (&buf)[2], const StageTw<N, NT, TPB, MAXR>& t,
                                                    LdIn ld_in, Pw pw, StOut st_out)
@@ -547,7 +467,7 @@ __device__ __forceinline__ void dft_square_pass_pp(double2* const (&buf)[2], con
         constexpr int H = pp_head_out<N, MAXR, 0>();  // the forward head's output
         dft_pp_head<N, NT, TPB, MAXR, false, 1, 0, 0>(buf, t, ld_in);
         // the turn: forward last stage from buf[H], inverse first stage into buf[1 - H]
-        constexpr int NS = LastStage<N, MAXR>::ns(), SI = LastStage<N, MAXR>::si();
+        constexpr int SI = LastStage<N, MAXR>::si();
         using S = StageShape<N, NT, TPB, P0>;
         const int tr = threadIdx.x % NT;
 #pragma unroll
@@ -569,7 +489,6 @@ __device__ __forceinline__ void dft_square_pass_pp(double2* const (&buf)[2], con
                 for (int r = 0; r < P0; ++r) buf[1 - H][Lay::at(tr, j * P0 + r)] = v[r];
             }
         }
-        (void)NS;
         __syncthreads();
         dft_pp_rest<N, NT, TPB, MAXR, true, P0, 1, H>(buf, t, ld_in, st_out);
     } else {
