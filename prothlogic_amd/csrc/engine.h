@@ -99,6 +99,11 @@ class Group {
     // fills per-slot maxima (capacity entries, nullable).
     void enter(uint32_t slot);
     double step(uint32_t steps, double* slot_err);
+    // step() in two halves, so that several groups' chunks run at once on
+    // their own streams: step_begin enqueues one chunk of g <= kStepChunk
+    // steps; step_end waits for it and reads its round-off like step().
+    void step_begin(uint32_t g);
+    double step_end(double* slot_err);
     double leave(uint32_t slot);
     std::vector<SlotState> states();
 
@@ -166,6 +171,7 @@ class Group {
     // plain squaring chunks, one graph per slot range (slot0 << 32 | slots)
     std::map<uint64_t, std::pair<hipGraphExec_t, hipGraph_t>> plain_graphs_;
     uint32_t sq0_ = 0, sqn_ = 1;
+    uint32_t pending_steps_ = 0;  // steps enqueued by step_begin, not yet read back
     bool uniform_ = true;  // every slot holds the plan's n and k
     bool turn_ = true;     // split outer pass as one launch (k_carry_turn); PL_CARRY_TURN=0: two
     bool give_up_ = false; // PL_HANDOFF_GIVE_UP=1: every carry hand-off wait gives up (tests)

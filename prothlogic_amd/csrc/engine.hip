@@ -1792,24 +1792,41 @@ void Group::enter(uint32_t slot)
     check(hipGetLastError(), "enter");
 }
 
+void Group::step_begin(uint32_t g)
+{
+    if (g == 0 || g > (uint32_t)kStepChunk) throw std::invalid_argument("step_begin: 1..kStepChunk steps");
+    if (pending_steps_) throw std::logic_error("step_begin: the previous chunk was not read back");
+    check(hipSetDevice(device_), "hipSetDevice");
+    if (g == (uint32_t)kStepChunk) {
+        if (!step_exec_) build_graph(step_exec_, step_graph_, true, kStepChunk);
+        check(hipGraphLaunch(step_exec_, stream_), "hipGraphLaunch");
+    } else {
+        enqueue_err_clear();
+        enqueue_step_chunk((int)g);
+        enqueue_err_reduce(0, cap_);
+        check(hipGetLastError(), "step launch");
+    }
+    pending_steps_ = g;
+}
+
+double Group::step_end(double* slot_err)
+{
+    if (!pending_steps_) throw std::logic_error("step_end: no chunk in flight");
+    check(hipSetDevice(device_), "hipSetDevice");
+    const int g = (int)pending_steps_;
+    pending_steps_ = 0;
+    return read_err(g, nullptr, slot_err);
+}
+
 double Group::step(uint32_t steps, double* slot_err)
 {
-    check(hipSetDevice(device_), "hipSetDevice");
     double emax = 0.0;
     std::vector<double> se(cap_, 0.0), tmp(cap_, 0.0);
     uint32_t done = 0;
     while (done < steps) {
-        const int g = (int)std::min<uint32_t>(kStepChunk, steps - done);
-        if (g == kStepChunk) {
-            if (!step_exec_) build_graph(step_exec_, step_graph_, true, kStepChunk);
-            check(hipGraphLaunch(step_exec_, stream_), "hipGraphLaunch");
-        } else {
-            enqueue_err_clear();
-            enqueue_step_chunk(g);
-            enqueue_err_reduce(0, cap_);
-            check(hipGetLastError(), "step launch");
-        }
-        emax = std::max(emax, read_err(g, nullptr, tmp.data()));
+        const uint32_t g = std::min<uint32_t>(kStepChunk, steps - done);
+        step_begin(g);
+        emax = std::max(emax, step_end(tmp.data()));
         for (uint32_t s = 0; s < cap_; ++s) se[s] = std::max(se[s], tmp[s]);
         done += g;
     }

@@ -247,19 +247,32 @@ int pl_pool_run(pl_pool* P, uint32_t steps, pl_pool_result* out, size_t cap, siz
     if (!P || (!out && cap) || !n_out) return pfail(PL_ERR_ARG, "NULL argument");
     try {
         admit(P);
+        // Every group with work enqueues a chunk on its own stream before any
+        // is read back, so the groups' kernels overlap on the device.
+        std::vector<LenGroup*> live;
         for (auto& kv : P->groups) {
-            LenGroup& lg = *kv.second;
             bool any = false;
-            for (auto& o : lg.slots) any = any || o.used;
-            if (!any) continue;
-            std::vector<double> se(lg.slots.size(), 0.0);
-            lg.g->step(steps, se.data());
-            if (lg.g->status()) {
-                lg.g->clear_status();
-                return pfail(PL_ERR_STATE, "device carry did not settle");
+            for (auto& o : kv.second->slots) any = any || o.used;
+            if (any) live.push_back(kv.second.get());
+        }
+        for (uint32_t done = 0; done < steps;) {
+            const uint32_t g = std::min<uint32_t>(pleng::Group::kStepChunk, steps - done);
+            for (LenGroup* lg : live) lg->g->step_begin(g);
+            for (LenGroup* lg : live) {
+                std::vector<double> se(lg->slots.size(), 0.0);
+                lg->g->step_end(se.data());
+                for (size_t s = 0; s < lg->slots.size(); ++s)
+                    if (lg->slots[s].used) lg->slots[s].max_err = std::max(lg->slots[s].max_err, se[s]);
             }
-            for (size_t s = 0; s < lg.slots.size(); ++s)
-                if (lg.slots[s].used) lg.slots[s].max_err = std::max(lg.slots[s].max_err, se[s]);
+            done += g;
+        }
+        for (LenGroup* lg : live) {
+            const int st = lg->g->status();
+            if (st) {
+                lg->g->clear_status();
+                return pfail(PL_ERR_STATE, st & 2 ? "carry hand-off between workgroups timed out"
+                                                  : "device carry did not settle");
+            }
         }
         retire(P);
         admit(P);
