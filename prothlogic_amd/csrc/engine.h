@@ -154,6 +154,7 @@ class Group {
     double2* z_ = nullptr;
     long long* carries_ = nullptr;
     long long* runc_ = nullptr;
+    unsigned* epoch_ = nullptr;   // cap x 2: next and last carry hand-off epoch
     int* rdig_ = nullptr;
     long long* xhi_ = nullptr;
     unsigned long long* err_ = nullptr;   // cap x kChunk (bit patterns of non-negative doubles)

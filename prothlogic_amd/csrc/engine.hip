@@ -84,6 +84,7 @@ struct Params {
     long long* carries;     // flat carry kernel: 3 per block (own carry-outs of its last run)
     long long* runc;        // split outer pass: 3 carry-outs per run (block-major, then row);
                             // the one-launch pass stores them as tagged words
+    unsigned* epoch;        // per slot: [0] the next hand-off epoch, [1] the last one used
     int* dig;               // split outer pass: NT int32 digits per run (same order)
     long long* xhi;
     const long long* xhi_mul;  // product pass: the other factor's X_hi (nullptr: squaring)
@@ -130,6 +131,7 @@ __device__ __forceinline__ Params bind(const Params& base, unsigned y)
     p.z += (size_t)slot * base.L;
     p.dig += (size_t)slot * base.L;
     p.runc += (size_t)slot * base.runc_stride;
+    p.epoch += 2 * (size_t)slot;
     p.carries += (size_t)slot * base.carr_stride;
     p.xhi += slot;
     p.err += (size_t)slot * base.err_stride;  // err_stride = blocks * kChunk
@@ -399,6 +401,8 @@ __global__ __launch_bounds__(TPB) void k_row(Params pb)
     const size_t base = (size_t)blockIdx.x * NT * C;
     const double2* __restrict__ z = p.z;
     const double x2 = 2.0 * (double)(*p.xhi);
+    // the next carry hand-off's epoch (k_carry_turn)
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.epoch[0] = p.epoch[1] + 1u;
     dft_square_pass_pp<C, NT, TPB, PL_ROW_RADIX>(
         buf, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
         [&](double2 X) { return cmul(X, make_double2(X.x, X.y + x2)); },
@@ -760,9 +764,12 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
 // nothing but these atomics crosses workgroups, so there is no L2 write-back
 // or invalidate (an agent-scope release/acquire pair made the launch four
 // times slower at L = 2^20, flushing the z lines other blocks had just
-// written) and no second round trip for a separate flag.  A run's epoch is
-// the tag its own words carry from the previous launch, plus one; all runs
-// of a slot go through the same launches, so they agree on it.  Every block
+// written) and no second round trip for a separate flag.  The epoch is one
+// word per slot, read with a scalar load (it does not queue in front of the
+// data loads): k_row sets it to the last hand-off's epoch plus one, and
+// k_carry_turn records the epoch it used.  Between two hand-offs of a slot
+// there is always a row pass, so consecutive hand-offs never share a tag,
+// however many row passes run without one.  Every block
 // publishes before it waits, a block waits only on blockIdx.x - 1 (block 0
 // on the slot's last block), and workgroups start in increasing order on
 // each XCD, so the waits cannot form a cycle; a wait still gives up after
@@ -814,15 +821,11 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         const unsigned j = threadIdx.x % NT;
         s_wc[threadIdx.x / NT][j] = threadIdx.x < NT ? p.wci[c0 + j] : p.wcf[c0 + j];
     }
+    const unsigned e = p.epoch[0] & kTagMask;
+    if (b == 0 && threadIdx.x == 0) p.epoch[1] = e;
     unsigned ep[RPT];
 #pragma unroll
-    for (int u = 0; u < RPT; ++u) {
-        const unsigned r = threadIdx.x + u * TPB;
-        ep[u] = r < (unsigned)R
-                    ? (tag_of(__hip_atomic_load(rw + (size_t)b * R + r, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT)) + 1u) & kTagMask
-                    : 0u;
-    }
+    for (int u = 0; u < RPT; ++u) ep[u] = e;
     double2 wri[RPT];
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
@@ -1377,6 +1380,7 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     const size_t o_z = off; off += al(sizeof(double2) * p.L * cap_);
     const size_t o_car = off; off += al(sizeof(long long) * 3 * carry_blocks_ * cap_);
     const size_t o_runc = off; off += al(sizeof(long long) * 3 * runs * cap_);
+    const size_t o_epoch = off; off += al(sizeof(unsigned) * 2 * cap_);
     const size_t o_rdig = off; off += al(p.nt ? sizeof(int) * (size_t)p.L * cap_ : 0);
     const size_t o_xhi = off; off += al(sizeof(long long) * cap_);
     eb_ = std::max<uint32_t>(carry_blocks_, p.nt ? p.C / p.nt : 1);
@@ -1409,6 +1413,7 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     z_ = reinterpret_cast<double2*>(b + o_z);
     carries_ = reinterpret_cast<long long*>(b + o_car);
     runc_ = reinterpret_cast<long long*>(b + o_runc);
+    epoch_ = reinterpret_cast<unsigned*>(b + o_epoch);
     rdig_ = reinterpret_cast<int*>(b + o_rdig);
     xhi_ = reinterpret_cast<long long*>(b + o_xhi);
     err_ = reinterpret_cast<unsigned long long*>(b + o_err);
@@ -1501,6 +1506,7 @@ Params Group::params(uint32_t slot0, bool check_active, uint32_t err_col) const
     p.wci = tabs_->wci;
     p.carries = carries_;
     p.runc = runc_;
+    p.epoch = epoch_;
     p.dig = rdig_;
     p.xhi = xhi_;
     p.xhi_mul = nullptr;
