@@ -21,22 +21,24 @@
 // squaring keep |x| below ~k*2^(n-1); the residue is therefore held in the
 // Montgomery form y = x*2^(2n) mod N (conversions happen on the host, at the
 // start and end of a test only).  The three carry passes (T_lo, the first fold
-// S, the second fold R) run per run of consecutive digits in int64; each run
-// publishes its own three carry-outs, computed from its own digits only, and
-// the next run folds them into its first digits (one hop, no chain).
+// S, the second fold R) run per run of consecutive digits, exactly in FP64;
+// each run publishes its own three carry-outs, computed from its own digits
+// only, and the next run folds them into its first digits (one hop, no
+// chain).
 //
 // FFT.  L = R*C, four-step on the R x C row-major array (digit m at r*C + c):
 // R-point DFTs down the columns with the W_L^(c*rho) twiddle, then C-point
 // DFTs along rows; the spectrum stays in that transposed order.  One squaring
 // is two kernels in steady state:
-//   k_inv_carry  : inverse column DFT -> unweight/round -> own carries of
-//                  each run of NT digits -> int32 digits + run carry-outs
-//   k_adjust_fwd : predecessor run's carries folded in -> weight -> forward
-//                  column DFT
+//   k_carry_turn : inverse column DFT -> unweight/round -> own carries of
+//                  each run of 8 digits -> carry hand-off to the next column
+//                  block -> predecessor's carries folded in -> weight ->
+//                  forward column DFT
 //   k_row        : forward row DFT -> Z*(Z + 2i*X_hi) -> inverse row DFT
-// k_fwd_col / k_inv_col / k_carry / k_fixup are the unfused pieces used to
-// enter and leave the steady state (and for lengths the split pass does not
-// cover).
+// k_inv_carry + k_adjust_fwd are the same outer pass in two launches (runs
+// of 16 digits, or PL_CARRY_TURN=0); k_fwd_col / k_inv_col / k_carry /
+// k_fixup are the unfused pieces used to enter and leave the steady state
+// (and for lengths the split pass does not cover).
 //
 // Slots.  Every launch covers the slots of a Group (blockIdx.y = slot); a
 // slot binds its own n, k, residue, carry staging, X_hi and round-off slot.
