@@ -771,17 +771,19 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
 // data loads): k_row sets it to the last hand-off's epoch plus one, and
 // k_carry_turn records the epoch it used.  Between two hand-offs of a slot
 // there is always a row pass, so consecutive hand-offs never share a tag,
-// however many row passes run without one.  Every block
-// publishes before it waits, a block waits only on blockIdx.x - 1 (block 0
-// on the slot's last block), and workgroups start in increasing order on
-// each XCD, so the waits cannot form a cycle; a wait still gives up after
-// kSpinLimit polls (status bit 2, well over 0.1 s where a normal wait is
-// microseconds) so that no launch can hang.  PL_HANDOFF_GIVE_UP=1 (tests of
-// that path) makes every wait expect a tag no run writes and give up after
-// 64 polls.  The words lie in three planes
-// (word i of run q at runc[i*runs + q]), so the 64 lanes of a wave store and
-// poll 512 contiguous bytes per instruction instead of touching each line
-// three times.
+// however many row passes run without one.
+//
+// Every block publishes before it waits, a block waits only on blockIdx.x - 1
+// (block 0 on the slot's last block), and workgroups start in increasing
+// order on each XCD, so the waits cannot form a cycle; a wait still gives up
+// after kSpinLimit polls (status bit 2, well over 0.1 s where a normal wait
+// is microseconds) so that no launch can hang.  PL_HANDOFF_GIVE_UP=1 (tests
+// of that path) makes every wait expect a tag no run writes and give up
+// after 64 polls.
+//
+// The words lie in three planes (word i of run q at runc[i*runs + q]), so the
+// 64 lanes of a wave store and poll 512 contiguous bytes per instruction
+// instead of touching each line three times.
 constexpr unsigned kSpinLimit = 1u << 18;
 constexpr int kTagBits = 10;
 constexpr unsigned kTagMask = (1u << kTagBits) - 1;
