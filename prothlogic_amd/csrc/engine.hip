@@ -403,12 +403,16 @@ __global__ __launch_bounds__(TPB) void k_row(Params pb)
     const size_t base = (size_t)blockIdx.x * NT * C;
     const double2* __restrict__ z = p.z;
     const double x2 = 2.0 * (double)(*p.xhi);
-    // the next carry hand-off's epoch (k_carry_turn)
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.epoch[0] = p.epoch[1] + 1u;
+    // The next carry hand-off's epoch (k_carry_turn), set by block 0.  Every
+    // lane reads the last one with a vector load that nothing waits for until
+    // the row is stored (a load in the writing lane's branch alone would be
+    // waited for there, in front of block 0's row loads).
+    const unsigned last = __hip_atomic_load(p.epoch + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     dft_square_pass_pp<C, NT, TPB, PL_ROW_RADIX>(
         buf, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
         [&](double2 X) { return cmul(X, make_double2(X.x, X.y + x2)); },
         [&](int tr, int idx, double2 v, int, int) { p.z[base + (size_t)tr * C + idx] = v; });
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.epoch[0] = last + 1u;
 }
 
 // Row pass of a product: this slot's row times the same row of another
@@ -767,9 +771,9 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
 // or invalidate (an agent-scope release/acquire pair made the launch four
 // times slower at L = 2^20, flushing the z lines other blocks had just
 // written) and no second round trip for a separate flag.  The epoch is one
-// word per slot, read with a scalar load (it does not queue in front of the
-// data loads): k_row sets it to the last hand-off's epoch plus one, and
-// k_carry_turn records the epoch it used.  Between two hand-offs of a slot
+// word per slot: k_row sets it to the last hand-off's epoch plus one (after
+// its row is stored), and k_carry_turn reads it with a scalar load and
+// records the epoch it used.  Between two hand-offs of a slot
 // there is always a row pass, so consecutive hand-offs never share a tag,
 // however many row passes run without one.
 //
@@ -821,6 +825,9 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     const size_t runs = (size_t)nb * R;
     StageTw<R, NT, TPB> t;
     load_stage_tw(t, p.tw);
+    // Staged at once, although first used after the inverse DFT: holding the
+    // weights in registers until then, or reading the epoch with a vector
+    // load, was measured slower on MI355X.
     if (threadIdx.x < 2 * NT) {
         const unsigned j = threadIdx.x % NT;
         s_wc[threadIdx.x / NT][j] = threadIdx.x < NT ? p.wci[c0 + j] : p.wcf[c0 + j];
