@@ -459,12 +459,17 @@ template <int R, int NT, int TPB, int P>
 struct ColTw {
     using S = StageShape<R, NT, TPB, P>;
     double2 w[S::PER][P];
-    __device__ __forceinline__ void load(const Params& p, unsigned c)
+    // SCALED: every factor also carries f, a factor that is constant down
+    // column c (k_carry_turn folds the column weight in here: the column DFT
+    // is linear, so scaling its input or output by f scales the whole column)
+    template <bool SCALED = false>
+    __device__ __forceinline__ void load(const Params& p, unsigned c, double2 f = make_double2(1.0, 0.0))
     {
 #pragma unroll
         for (int u = 0; u < S::PER; ++u) {
             if (!S::live(u)) continue;
-            const double2 a = twL(p, c * (unsigned)S::j(u));
+            double2 a = twL(p, c * (unsigned)S::j(u));
+            if constexpr (SCALED) a = cmul(a, f);
             const double2 q = p.tcq[(size_t)c * 16 + 16 / P];
             w[u][0] = a;
             double2 qr = q;
@@ -815,7 +820,6 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     __shared__ double2 lds[PP ? 2 : 1][Lay::kSlots];
     double2* const buf[2] = { lds[0], lds[PP ? 1 : 0] };
     double2* const io = buf[PP ? IO : 0];
-    __shared__ double2 s_wc[2][NT];  // the block's column weight factors: inverse, forward
     __shared__ double s_err[TPB / 64];
     const unsigned nb = p.C / NT;
     const unsigned b = blockIdx.x;
@@ -825,13 +829,6 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     const size_t runs = (size_t)nb * R;
     StageTw<R, NT, TPB> t;
     load_stage_tw(t, p.tw);
-    // Staged at once, although first used after the inverse DFT: holding the
-    // weights in registers until then, or reading the epoch with a vector
-    // load, was measured slower on MI355X.
-    if (threadIdx.x < 2 * NT) {
-        const unsigned j = threadIdx.x % NT;
-        s_wc[threadIdx.x / NT][j] = threadIdx.x < NT ? p.wci[c0 + j] : p.wcf[c0 + j];
-    }
     const unsigned e = p.epoch[0] & kTagMask;
     if (b == 0 && threadIdx.x == 0) p.epoch[1] = e;
     unsigned ep[RPT];
@@ -844,8 +841,11 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         wri[u] = r < (unsigned)R ? p.wri[r] : make_double2(0.0, 0.0);
     }
     {
+        // the inverse column weight rides on the four-step factors: the data
+        // is multiplied by conj(a*conj(wci)*q^r) = conj(a*q^r)*wci
         ColTw<R, NT, TPB, P0> ct;
-        ct.load(p, c0 + threadIdx.x % NT);
+        const double2 wc = p.wci[c0 + threadIdx.x % NT];
+        ct.template load<true>(p, c0 + threadIdx.x % NT, make_double2(wc.x, -wc.y));
         auto ld = [&](int tr, int idx, int u, int r) { return cmulc(z[(size_t)idx * p.C + c0 + tr], ct.w[u][r]); };
         auto st = [&](int tr, int idx, double2 v, int, int) { io[Lay::at(tr, idx)] = v; };
         if constexpr (PP) dft_pp<R, NT, TPB, true, 0>(buf, t, ld, st);
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         double q1 = 0, q2 = 0, q3 = 0;
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
-            double2 w = cmul(wri[u], s_wc[0][j]);
+            double2 w = wri[u];
             unsigned f = F + G[j];
             const bool wrap = f >= p.L;
             if (wrap) {
@@ -893,9 +893,10 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     emax = wave_max(emax);
     if ((threadIdx.x & 63) == 0) s_err[threadIdx.x / 64] = emax;
 
-    // forward-side tables, requested while the predecessors finish
+    // forward-side tables (the forward column weight folded into the
+    // four-step factors), requested while the predecessors finish
     ColTw<R, NT, TPB, PL> ct;
-    ct.load(p, c0 + threadIdx.x % NT);
+    ct.template load<true>(p, c0 + threadIdx.x % NT, p.wcf[c0 + threadIdx.x % NT]);
     double2 wrf[RPT];
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
@@ -943,7 +944,7 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         const unsigned F = (unsigned)((0ull - p.n * (unsigned long long)r * p.C) & Lm);
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
-            double2 w = cmul(wrf[u], s_wc[1][j]);
+            double2 w = wrf[u];
             if (F + G[j] >= p.L) w = make_double2(0.5 * w.x, 0.5 * w.y);
             const double xd = (double)dg[u][j] + (double)nu[j];
             io[Lay::at(j, r)] = make_double2(xd * w.x, xd * w.y);
