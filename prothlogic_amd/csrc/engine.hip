@@ -946,7 +946,9 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         for (int j = 0; j < NT; ++j) {
             double2 w = wrf[u];
             if (F + G[j] >= p.L) w = make_double2(0.5 * w.x, 0.5 * w.y);
-            const double xd = (double)dg[u][j] + (double)nu[j];
+            // digit and adjustment are balanced digits (|.| <= 2^b/2, b < 32):
+            // their sum is exact in 32-bit integers
+            const double xd = (double)(dg[u][j] + (int)nu[j]);
             io[Lay::at(j, r)] = make_double2(xd * w.x, xd * w.y);
         }
     }
