@@ -890,6 +890,22 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         __hip_atomic_store(w + runs, tag_word(q2, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(w + 2 * runs, tag_word(q3, ep[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    // Predecessor of run (b, r): (b-1, r); (nb-1, r-1) for b = 0; run (0, 0)
+    // takes the top run (nb-1, R-1) through the top rule.  The first poll of
+    // its words goes out now, so that the round-off reduction's lane
+    // exchanges and the forward-side tables overlap its round trip.
+    const unsigned long long* pw[RPT];
+    unsigned long long w0[RPT], w1[RPT], w2[RPT];
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        if (r >= (unsigned)R) break;
+        pw[u] = rw + (b == 0 ? (r == 0 ? (size_t)(nb - 1) * R + (R - 1) : (size_t)(nb - 1) * R + (r - 1))
+                             : (size_t)(b - 1) * R + r);
+        w0[u] = __hip_atomic_load(pw[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w1[u] = __hip_atomic_load(pw[u] + runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w2[u] = __hip_atomic_load(pw[u] + 2 * runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     emax = wave_max(emax);
     if ((threadIdx.x & 63) == 0) s_err[threadIdx.x / 64] = emax;
 
@@ -904,8 +920,7 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         wrf[u] = r < (unsigned)R ? p.wrf[r] : make_double2(0.0, 0.0);
     }
 
-    // Predecessor of run (b, r): (b-1, r); (nb-1, r-1) for b = 0; run (0, 0)
-    // takes the top run (nb-1, R-1) through the top rule.  Each thread
+    // Poll until the three words carry this launch's tag.  Each thread
     // rewrites only the LDS slots of its own runs, which only it read in the
     // carry phase: no barrier before the weighted digits go in.
     bool late = false;
@@ -913,24 +928,19 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     for (int u = 0; u < RPT; ++u) {
         const unsigned r = threadIdx.x + u * TPB;
         if (r >= (unsigned)R) break;
-        const unsigned long long* pw =
-            rw + (b == 0 ? (r == 0 ? (size_t)(nb - 1) * R + (R - 1) : (size_t)(nb - 1) * R + (r - 1))
-                         : (size_t)(b - 1) * R + r);
-        unsigned long long w0, w1, w2;
+        const unsigned want = (ep[u] + p.tag_skew) & kTagMask;
         unsigned spins = 0;
-        for (;;) {
-            w0 = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            w1 = __hip_atomic_load(pw + runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            w2 = __hip_atomic_load(pw + 2 * runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned want = (ep[u] + p.tag_skew) & kTagMask;
-            if (tag_of(w0) == want && tag_of(w1) == want && tag_of(w2) == want) break;
+        while (!(tag_of(w0[u]) == want && tag_of(w1[u]) == want && tag_of(w2[u]) == want)) {
             if (++spins >= p.spin_limit) {
                 late = true;
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
+            w0[u] = __hip_atomic_load(pw[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            w1[u] = __hip_atomic_load(pw[u] + runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            w2[u] = __hip_atomic_load(pw[u] + 2 * runs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        const long long cin[3] = { untag(w0), untag(w1), untag(w2) };
+        const long long cin[3] = { untag(w0[u]), untag(w1[u]), untag(w2[u]) };
         __int128 c2in, extra;
         if (b == 0 && r == 0) {
             top_rule(p, cin, c2in, extra);
