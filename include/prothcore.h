@@ -266,7 +266,10 @@ typedef struct pl_pool_result {
 } pl_pool_result;
 
 /* hbm_budget_bytes = 0: 64 GiB; max_slots_per_length = 0: 256;
- * k_max = 0: 2^32-1 (smaller values allow the 8-digit carry runs). */
+ * k_max = 0: 2^32-1.  A length group's carry runs must absorb carries at the
+ * narrowest digits the length admits (about 9 bits); they are 8 digits long
+ * (the one-launch outer pass) only for small k_max (below 2^12 at
+ * L <= 2^15, 2^11 at L = 2^16..2^17), else 16 (the split outer pass). */
 int pl_pool_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots_per_length, uint64_t k_max,
                    pl_pool** out);
 void pl_pool_destroy(pl_pool* pool);
