@@ -102,7 +102,11 @@ class Group {
     // step() in two halves, so that several groups' chunks run at once on
     // their own streams: step_begin enqueues one chunk of g <= kStepChunk
     // steps; step_end waits for it and reads its round-off like step().
-    void step_begin(uint32_t g);
+    // slots > 0: only slots [0, slots) can be active, so the chunk's launches
+    // cover the smallest power of two >= max(slots, 8) of them (capped at
+    // the capacity) instead of every slot; step_end then reports round-off
+    // for those slots only (0 for the rest).
+    void step_begin(uint32_t g, uint32_t slots = 0);
     double step_end(double* slot_err);
     double leave(uint32_t slot);
     std::vector<SlotState> states();
@@ -141,8 +145,8 @@ class Group {
     void enqueue_plain_chunk(int g);
     void enqueue_step_chunk(int g);
     void build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g);
-    double read_err(int g, double* trace, double* slot_err, uint32_t trace_slot = 0);
-    void enqueue_err_clear();
+    double read_err(int g, double* trace, double* slot_err, uint32_t trace_slot = 0, uint32_t nslots = 0);
+    void enqueue_err_clear(uint32_t slots);
     void enqueue_err_reduce(uint32_t slot0, uint32_t slots);
 
     Plan plan_;
@@ -167,8 +171,9 @@ class Group {
     DeviceTables* tabs_ = nullptr;
     uint32_t carry_blocks_ = 0, carry_tpb_ = 64;
     std::vector<SlotDesc> desc_host_;
-    hipGraphExec_t step_exec_ = nullptr;
-    hipGraph_t step_graph_ = nullptr;
+    // scheduled-step chunks, one graph per launched slot count
+    std::map<uint32_t, std::pair<hipGraphExec_t, hipGraph_t>> step_graphs_;
+    uint32_t step_n_ = 0;  // slots the step chunk being enqueued launches
     // plain squaring chunks, one graph per slot range (slot0 << 32 | slots)
     std::map<uint64_t, std::pair<hipGraphExec_t, hipGraph_t>> plain_graphs_;
     uint32_t sq0_ = 0, sqn_ = 1;

@@ -1503,8 +1503,10 @@ Group::~Group()
         (void)hipGraphExecDestroy(kv.second.first);
         (void)hipGraphDestroy(kv.second.second);
     }
-    if (step_exec_) (void)hipGraphExecDestroy(step_exec_);
-    if (step_graph_) (void)hipGraphDestroy(step_graph_);
+    for (auto& kv : step_graphs_) {
+        (void)hipGraphExecDestroy(kv.second.first);
+        (void)hipGraphDestroy(kv.second.second);
+    }
     if (mem_) (void)hipFree(mem_);
     if (stream_) (void)hipStreamDestroy(stream_);
     delete tabs_;
@@ -1574,7 +1576,7 @@ void Group::launch(int kind, const Params& p, uint32_t slots)
         break;
     }
     case kAdvance:
-        hipLaunchKernelGGL(k_advance, dim3((cap_ + 63) / 64), dim3(64), 0, stream_, state_, desc_, cap_);
+        hipLaunchKernelGGL(k_advance, dim3((slots + 63) / 64), dim3(64), 0, stream_, state_, desc_, slots);
         break;
     default: dispatch_col(kind, p, slots, plan_.nt, stream_);
     }
@@ -1685,37 +1687,38 @@ void Group::enqueue_step_chunk(int g)
     for (int i = 0; i < g; ++i) {
         const Params p = params(0, true, (uint32_t)i);
         if (turn()) {
-            launch(kCarryTurn, p, cap_);
+            launch(kCarryTurn, p, step_n_);
         } else if (plan_.nt) {
-            launch(kInvCarry, p, cap_);
-            launch(kAdjFwd, p, cap_);
+            launch(kInvCarry, p, step_n_);
+            launch(kAdjFwd, p, step_n_);
         } else {
-            launch(kInvCol, p, cap_);
-            launch(kCarry, p, cap_);
-            launch(kFixup, p, cap_);
-            launch(kFwdCol, p, cap_);
+            launch(kInvCol, p, step_n_);
+            launch(kCarry, p, step_n_);
+            launch(kFixup, p, step_n_);
+            launch(kFwdCol, p, step_n_);
         }
-        launch(kRow, p, cap_);
-        launch(kAdvance, p, cap_);
+        launch(kRow, p, step_n_);
+        launch(kAdvance, p, step_n_);
     }
 }
 
 void Group::build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g)
 {
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    enqueue_err_clear();
+    const uint32_t ns = stepping ? step_n_ : cap_;
+    enqueue_err_clear(ns);
     if (stepping) enqueue_step_chunk(g);
     else enqueue_plain_chunk(g);
-    enqueue_err_reduce(0, cap_);
+    enqueue_err_reduce(0, ns);
     check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
     check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
 }
 
 // Zero the round-off partials of every slot (a fill kernel, so the chunk
 // can be graph-captured).
-void Group::enqueue_err_clear()
+void Group::enqueue_err_clear(uint32_t slots)
 {
-    const int cnt = (int)((size_t)kChunk * eb_ * cap_);
+    const int cnt = (int)((size_t)kChunk * eb_ * slots);
     hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, errp_, cnt);
 }
 
@@ -1726,8 +1729,9 @@ void Group::enqueue_err_reduce(uint32_t slot0, uint32_t slots)
 
 // Reads the chunk's err columns: trace gets slot 0's per-column values,
 // slot_err each slot's max; returns the max over everything.
-double Group::read_err(int g, double* trace, double* slot_err, uint32_t trace_slot)
+double Group::read_err(int g, double* trace, double* slot_err, uint32_t trace_slot, uint32_t nslots)
 {
+    const uint32_t live = nslots ? nslots : cap_;
     check(hipMemcpyAsync(err_host_.data(), err_, sizeof(unsigned long long) * kChunk * cap_, hipMemcpyDeviceToHost,
                          stream_),
           "err D2H");
@@ -1735,7 +1739,7 @@ double Group::read_err(int g, double* trace, double* slot_err, uint32_t trace_sl
     double emax = 0.0;
     for (uint32_t s = 0; s < cap_; ++s) {
         double smax = 0.0;
-        for (int i = 0; i < g; ++i) {
+        for (int i = 0; i < g && s < live; ++i) {
             double e;
             std::memcpy(&e, &err_host_[(size_t)s * kChunk + i], sizeof e);
             if (s == trace_slot && trace) trace[i] = e;
@@ -1770,7 +1774,7 @@ double Group::square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, doubl
             }
             check(hipGraphLaunch(it->second.first, stream_), "hipGraphLaunch");
         } else {
-            enqueue_err_clear();
+            enqueue_err_clear(cap_);
             enqueue_plain_chunk(g);
             enqueue_err_reduce(0, cap_);
             check(hipGetLastError(), "square launch");
@@ -1830,18 +1834,32 @@ void Group::enter(uint32_t slot)
     check(hipGetLastError(), "enter");
 }
 
-void Group::step_begin(uint32_t g)
+void Group::step_begin(uint32_t g, uint32_t slots)
 {
     if (g == 0 || g > (uint32_t)kStepChunk) throw std::invalid_argument("step_begin: 1..kStepChunk steps");
     if (pending_steps_) throw std::logic_error("step_begin: the previous chunk was not read back");
+    if (slots > cap_) throw std::invalid_argument("step_begin: more slots than the group holds");
     check(hipSetDevice(device_), "hipSetDevice");
+    uint32_t ns = cap_;
+    if (slots) {
+        ns = 8;
+        while (ns < slots) ns *= 2;
+        ns = std::min(ns, cap_);
+    }
+    step_n_ = ns;
     if (g == (uint32_t)kStepChunk) {
-        if (!step_exec_) build_graph(step_exec_, step_graph_, true, kStepChunk);
-        check(hipGraphLaunch(step_exec_, stream_), "hipGraphLaunch");
+        auto it = step_graphs_.find(ns);
+        if (it == step_graphs_.end()) {
+            hipGraphExec_t ex = nullptr;
+            hipGraph_t gr = nullptr;
+            build_graph(ex, gr, true, kStepChunk);
+            it = step_graphs_.emplace(ns, std::make_pair(ex, gr)).first;
+        }
+        check(hipGraphLaunch(it->second.first, stream_), "hipGraphLaunch");
     } else {
-        enqueue_err_clear();
+        enqueue_err_clear(ns);
         enqueue_step_chunk((int)g);
-        enqueue_err_reduce(0, cap_);
+        enqueue_err_reduce(0, ns);
         check(hipGetLastError(), "step launch");
     }
     pending_steps_ = g;
@@ -1853,7 +1871,7 @@ double Group::step_end(double* slot_err)
     check(hipSetDevice(device_), "hipSetDevice");
     const int g = (int)pending_steps_;
     pending_steps_ = 0;
-    return read_err(g, nullptr, slot_err);
+    return read_err(g, nullptr, slot_err, 0, step_n_);
 }
 
 double Group::step(uint32_t steps, double* slot_err)

@@ -248,16 +248,24 @@ int pl_pool_run(pl_pool* P, uint32_t steps, pl_pool_result* out, size_t cap, siz
     try {
         admit(P);
         // Every group with work enqueues a chunk on its own stream before any
-        // is read back, so the groups' kernels overlap on the device.
+        // is read back, so the groups' kernels overlap on the device.  A
+        // chunk launches only the group's slots up to its highest occupied
+        // one (admission fills the lowest free slot first), so a sparsely
+        // filled group does not dispatch blocks for all its capacity.
         std::vector<LenGroup*> live;
+        std::vector<uint32_t> span;
         for (auto& kv : P->groups) {
-            bool any = false;
-            for (auto& o : kv.second->slots) any = any || o.used;
-            if (any) live.push_back(kv.second.get());
+            uint32_t hi = 0;
+            for (size_t s = 0; s < kv.second->slots.size(); ++s)
+                if (kv.second->slots[s].used) hi = (uint32_t)s + 1;
+            if (hi) {
+                live.push_back(kv.second.get());
+                span.push_back(hi);
+            }
         }
         for (uint32_t done = 0; done < steps;) {
             const uint32_t g = std::min<uint32_t>(pleng::Group::kStepChunk, steps - done);
-            for (LenGroup* lg : live) lg->g->step_begin(g);
+            for (size_t i = 0; i < live.size(); ++i) live[i]->g->step_begin(g, span[i]);
             for (LenGroup* lg : live) {
                 std::vector<double> se(lg->slots.size(), 0.0);
                 lg->g->step_end(se.data());
