@@ -91,3 +91,45 @@ def test_proth_test_seed_on_device(gpu):
         r = gpu.proth_test(k, n, stop_after=150)
         a = G.reference_base(k, n)
         assert r["res64"] == gmp_res64(k, n, a, 150), (k, n)
+
+
+@pytest.mark.parametrize("k_max", [1 << 11, 1 << 16])
+def test_pool_launch_span_follows_occupancy(gpu, k_max):
+    # A step chunk launches only the slots up to the highest occupied one,
+    # rounded up to a power of two (at least 8).  Results must not depend on
+    # the span a step ran with while it grows past 8 and 16 and holes open
+    # below it.  n = 200000 gives L = 2^14; k_max = 2^11 keeps the group's
+    # carry runs at 8 digits (the one-launch outer pass per slot), 2^16 makes
+    # them 16 (the split outer pass).
+    rnd = random.Random(11)
+    n = 200000
+
+    def cand():
+        while True:
+            k = 2 * rnd.randrange(1, k_max // 2) + 1
+            if not gpu.gate_rejects(k, n):
+                return k
+
+    first = [(cand(), 60) for _ in range(5)]      # retire early, leaving holes at the bottom
+    second = [(cand(), 400) for _ in range(20)]   # grows the span past 16
+    third = [(cand(), 200) for _ in range(6)]     # refills the holes
+    allc = first + second + third
+    print(f"[pool span] k_max={k_max} n={n} tests={len(allc)}", flush=True)
+    with gpu.Pool(k_max=k_max) as pool:
+        for i, (k, s) in enumerate(first):
+            pool.submit(k, n, tag=i, stop_after=s)
+        res = pool.run(steps=20)
+        for i, (k, s) in enumerate(second, start=len(first)):
+            pool.submit(k, n, tag=i, stop_after=s)
+        res += pool.run(steps=64)
+        res += pool.run(steps=64)
+        for i, (k, s) in enumerate(third, start=len(first) + len(second)):
+            pool.submit(k, n, tag=i, stop_after=s)
+        res += pool.drain(steps=64)
+    assert sorted(r["tag"] for r in res) == list(range(len(allc)))
+    for r in res:
+        k, s = allc[r["tag"]]
+        assert r["fft_len"] == 1 << 14
+        a = G.reference_base(k, n)
+        assert r["res64"] == gmp_res64(k, n, a, s), (k, n, s)
+        assert r["max_err"] < 0.3
