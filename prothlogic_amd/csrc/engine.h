@@ -119,6 +119,13 @@ class Group {
     // the one-launch outer pass is used (runs of 8 digits; with 16 its
     // registers would spill)
     bool turn() const { return turn_ && plan_.nt == 8; }
+    // Use the split outer pass even where the one-launch pass would run.  The
+    // one-launch pass's block 0 waits on its slot's last block, which starts
+    // after it: a launch only finishes when all its blocks can eventually be
+    // resident.  Kernels of other streams that spin the same way can hold
+    // the CUs that block needs, so callers that overlap groups on several
+    // streams (the pool) take the split pass.  Call before the first step.
+    void no_turn() { turn_ = false; }
 
     // Average duration (ms) of each kernel over all slots, timed with HIP
     // events between consecutive kernels of `reps` real plain squarings on

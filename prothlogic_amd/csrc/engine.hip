@@ -782,13 +782,17 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
 // there is always a row pass, so consecutive hand-offs never share a tag,
 // however many row passes run without one.
 //
-// Every block publishes before it waits, a block waits only on blockIdx.x - 1
-// (block 0 on the slot's last block), and workgroups start in increasing
-// order on each XCD, so the waits cannot form a cycle; a wait still gives up
-// after kSpinLimit polls (status bit 2, well over 0.1 s where a normal wait
-// is microseconds) so that no launch can hang.  PL_HANDOFF_GIVE_UP=1 (tests
-// of that path) makes every wait expect a tag no run writes and give up
-// after 64 polls.
+// Every block publishes before it waits and waits only on blockIdx.x - 1,
+// except block 0, which waits on the slot's last block; workgroups start in
+// increasing order on each XCD.  In a launch that has the device to itself,
+// every block but block 0 waits on words published before it started, so
+// the blocks ahead of a slot's last block finish and make room for it, and
+// block 0's wait ends.  Kernels of other streams that spin the same way can
+// take that room instead (Group::no_turn is for callers that overlap them).
+// A wait therefore still gives up after kSpinLimit polls (status bit 2, well
+// over 0.1 s where a normal wait is microseconds), so that no launch can
+// hang.  PL_HANDOFF_GIVE_UP=1 (tests of that path) makes every wait expect a
+// tag no run writes and give up after 64 polls.
 //
 // The words lie in three planes (word i of run q at runc[i*runs + q]), so the
 // 64 lanes of a wave store and poll 512 contiguous bytes per instruction

@@ -103,6 +103,7 @@ LenGroup* group_for(pl_pool* P, uint32_t n, uint64_t k, std::string& why)
     auto lg = std::make_unique<LenGroup>();
     lg->plan = pl;
     lg->g = std::make_unique<pleng::Group>(pl, cap, P->device);
+    lg->g->no_turn();  // groups step concurrently on their own streams
     lg->slots.resize(cap);
     P->used_bytes += lg->g->device_bytes();
     LenGroup* raw = lg.get();

@@ -99,8 +99,8 @@ def test_pool_launch_span_follows_occupancy(gpu, k_max):
     # rounded up to a power of two (at least 8).  Results must not depend on
     # the span a step ran with while it grows past 8 and 16 and holes open
     # below it.  n = 200000 gives L = 2^14; k_max = 2^11 keeps the group's
-    # carry runs at 8 digits (the one-launch outer pass per slot), 2^16 makes
-    # them 16 (the split outer pass).
+    # carry runs at 8 digits, 2^16 makes them 16 (both on the split outer
+    # pass, which the pool uses).
     rnd = random.Random(11)
     n = 200000
 
