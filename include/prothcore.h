@@ -9,7 +9,13 @@
  * The residue of a context stays on the device between calls; the host
  * sees canonical residues only through pl_get_residue / pl_res64 /
  * checkpoints.  A context owns one HIP stream (default priority) and is not
- * safe to use from two threads at once; distinct contexts are independent.
+ * safe to use from two threads at once; distinct contexts are independent
+ * and may run concurrently from several threads on one device (one test per
+ * worker thread, as P/dcore.cpp:125-192 dispatches): at most one of them at
+ * a time runs the one-launch outer pass (whose workgroups wait on each
+ * other), the others take the two-launch pass meanwhile, and a chunk whose
+ * carry hand-off still gives up (kernels of another process holding the
+ * CUs) is recomputed from its start on the two-launch pass.
  *
  * Each declaration names the reference interface it replaces
  * (P/ = prothlogic4.2/ in the reference tree).
@@ -31,7 +37,7 @@ extern "C" {
 #define PL_ERR_STATE (-4)    /* device carry did not settle (internal bug)  */
 #define PL_ERR_IO (-5)       /* checkpoint file could not be read/written   */
 
-#define PL_ABI_VERSION 3
+#define PL_ABI_VERSION 4
 
 typedef struct pl_ctx pl_ctx;
 
@@ -46,6 +52,8 @@ typedef struct pl_info {
     uint32_t passes;       /* kernels per steady-state squaring: 2 with the
                               one-launch outer pass, 3 with it in two
                               launches (PL_CARRY_TURN=0), 5 unfused          */
+    uint32_t handoff_fallbacks; /* chunks recomputed on the two-launch pass
+                              after a carry hand-off wait gave up            */
 } pl_info;
 
 /* Thread-local description of the last error (never NULL). */
@@ -73,7 +81,8 @@ int pl_autotune_fft_len(uint32_t n, uint64_t k, int device, double err_guard, ui
  * P/fft_engine.h:10-11).  fft_len = 0 picks pl_plan_fft_len's length.
  * The residue starts at 0.  Environment (read at creation): PL_CARRY_TURN=0
  * runs the outer pass as two launches (3 kernels per squaring instead of 2);
- * PL_FUSED=0 runs it unfused (5 kernels).  pl_info.passes reports which. */
+ * PL_FUSED=0 runs it unfused (5 kernels).  pl_info.passes reports which
+ * form the context prefers. */
 int pl_ctx_create(uint64_t k, uint32_t n, uint32_t fft_len, int device, pl_ctx** out);
 void pl_ctx_destroy(pl_ctx* ctx);
 int pl_ctx_info(const pl_ctx* ctx, pl_info* out);
@@ -253,7 +262,8 @@ typedef struct pl_pool_result {
     uint64_t k;
     uint32_t n;
     uint32_t fft_len;
-    int verdict;        /* 1 prime, 0 composite, -1 stopped early (stop_after) */
+    int verdict;        /* 1 prime, 0 composite, -1 stopped early (stop_after),
+                           -2 void (see error)                                 */
     int gated;          /* 1: rejected by the small-prime gate, no squarings   */
     uint64_t base;      /* a                                                   */
     uint64_t res64;     /* low 64 bits of a^(k*2^s) mod N, s = squarings after
@@ -263,13 +273,18 @@ typedef struct pl_pool_result {
     double t_submit;    /* seconds since the pool was created                  */
     double t_admit;
     double t_done;
+    int error;          /* PL_OK; else the result is void (verdict -2, no
+                           Res64): PL_ERR_ROUNDOFF when the round-off passed
+                           0.40, PL_ERR_STATE when the device carry did not
+                           settle in two restarts of the test from its seed  */
 } pl_pool_result;
 
 /* hbm_budget_bytes = 0: 64 GiB; max_slots_per_length = 0: 256;
- * k_max = 0: 2^32-1.  A length group's carry runs must absorb carries at the
- * narrowest digits the length admits (about 9 bits); they are 8 digits long
- * (the one-launch outer pass) only for small k_max (below 2^12 at
- * L <= 2^15, 2^11 at L = 2^16..2^17), else 16 (the split outer pass). */
+ * k_max = 0: 2^32-1.  Pool groups overlap on their own streams and always
+ * take the two-launch outer pass (no workgroup waits on another).  A length
+ * group's carry runs must absorb carries at the narrowest digits the length
+ * admits (about 9 bits): k_max chooses runs of 8 digits (below 2^12 at
+ * L <= 2^15, 2^11 at L = 2^16..2^17) or else 16. */
 int pl_pool_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots_per_length, uint64_t k_max,
                    pl_pool** out);
 void pl_pool_destroy(pl_pool* pool);
@@ -278,7 +293,9 @@ void pl_pool_destroy(pl_pool* pool);
 int pl_pool_submit(pl_pool* pool, uint64_t k, uint32_t n, uint64_t tag, uint64_t stop_after);
 /* Admit queued candidates, advance every active test by `steps` squarings
  * (fewer for tests that finish), retire finished tests and copy up to `cap`
- * results into out (*n_out of them; the rest stay for the next call). */
+ * results into out (*n_out of them; the rest stay for the next call).  A
+ * length group whose device carry did not settle has every test it holds
+ * restarted from its seed (or reported void, pl_pool_result.error). */
 int pl_pool_run(pl_pool* pool, uint32_t steps, pl_pool_result* out, size_t cap, size_t* n_out);
 int pl_pool_status(pl_pool* pool, size_t* queued, size_t* active, size_t* finished_unread, uint64_t* hbm_bytes);
 const char* pl_pool_last_error(void);

@@ -65,8 +65,8 @@ def build_host() -> Path | None:
     # the mirror's self-test program (tests/test_host_mirror.py runs it)
     exe = PKG / "prothhost_selftest"
     tmp = PKG / "prothhost_selftest.tmp"
-    _run([cxx, "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{HOST}", "-o", tmp, HOST / "selftest.cpp",
-          f"-L{PKG}", "-lprothhost", "-lprothcore", "-Wl,-rpath,$ORIGIN"])
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-pthread", f"-I{INCLUDE}", f"-I{HOST}", "-o", tmp,
+          HOST / "selftest.cpp", f"-L{PKG}", "-lprothhost", "-lprothcore", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, exe)
     # the command-line front end
     exe = PKG / "prothlogic"

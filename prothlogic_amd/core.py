@@ -32,7 +32,8 @@ class PlError(RuntimeError):
 class PlInfo(ctypes.Structure):
     _fields_ = [("k", ctypes.c_uint64), ("n", ctypes.c_uint32), ("fft_len", ctypes.c_uint32),
                 ("rows", ctypes.c_uint32), ("cols", ctypes.c_uint32), ("bits_per_digit", ctypes.c_double),
-                ("device_bytes", ctypes.c_uint64), ("passes", ctypes.c_uint32)]
+                ("device_bytes", ctypes.c_uint64), ("passes", ctypes.c_uint32),
+                ("handoff_fallbacks", ctypes.c_uint32)]
 
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double)
@@ -59,7 +60,8 @@ class PoolResult(ctypes.Structure):
     _fields_ = [("tag", ctypes.c_uint64), ("k", ctypes.c_uint64), ("n", ctypes.c_uint32), ("fft_len", ctypes.c_uint32),
                 ("verdict", ctypes.c_int), ("gated", ctypes.c_int), ("base", ctypes.c_uint64),
                 ("res64", ctypes.c_uint64), ("squarings", ctypes.c_uint64), ("max_err", ctypes.c_double),
-                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("error", ctypes.c_int)]
 
 
 _LIB = None
@@ -194,6 +196,12 @@ class Context:
     @property
     def fft_len(self) -> int:
         return self.info.fft_len
+
+    def refresh_info(self) -> PlInfo:
+        info = PlInfo()
+        _check(lib().pl_ctx_info(self.h, ctypes.byref(info)))
+        self.info = info
+        return info
 
     def set_residue(self, x: int):
         nl = max(1, (x.bit_length() + 63) // 64)

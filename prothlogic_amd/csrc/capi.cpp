@@ -7,6 +7,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <fcntl.h>
+#include <unistd.h>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -191,6 +193,7 @@ int pl_ctx_info(const pl_ctx* ctx, pl_info* out)
     out->bits_per_digit = (double)ctx->plan.n / (double)ctx->plan.L;
     out->device_bytes = ctx->eng->device_bytes();
     out->passes = (uint32_t)ctx->eng->launches_per_squaring();
+    out->handoff_fallbacks = ctx->eng->handoff_fallbacks();
     return PL_OK;
 }
 
@@ -378,11 +381,11 @@ uint64_t pl_choose_base(uint64_t k, uint32_t n)
 
 // ------------------------------------------------------------ checkpoints
 
-static uint64_t fnv1a(const uint64_t* p, size_t n)
+// FNV-1a over a byte range, continuing from h.
+static uint64_t fnv1a(const void* p, size_t bytes, uint64_t h = 1469598103934665603ull)
 {
-    uint64_t h = 1469598103934665603ull;
-    const unsigned char* b = reinterpret_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n * 8; ++i) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < bytes; ++i) {
         h ^= b[i];
         h *= 1099511628211ull;
     }
@@ -401,6 +404,17 @@ struct CkptV2 {
 };
 static_assert(sizeof(CkptV2) == 56 || sizeof(CkptV2) == 64, "header layout");
 
+// The checksum covers the whole header (with the checksum field taken as 0)
+// and the limbs, so a flipped k, n, base or next_iter is caught like a
+// flipped limb.
+static uint64_t ckpt_checksum(CkptV2 h, const uint64_t* limbs, size_t n_limbs)
+{
+    h.checksum = 0;
+    unsigned char hdr[64] = { 0 };
+    std::memcpy(hdr, &h, sizeof h);
+    return fnv1a(limbs, 8 * n_limbs, fnv1a(hdr, 64));
+}
+
 int pl_save_checkpoint(const char* path, uint64_t k, uint32_t n, uint64_t base, uint64_t next_iter,
                        const uint64_t* limbs, size_t n_limbs)
 {
@@ -413,17 +427,36 @@ int pl_save_checkpoint(const char* path, uint64_t k, uint32_t n, uint64_t base, 
     h.base = (uint32_t)base;
     h.next_iter = next_iter;
     h.n_limbs = n_limbs;
-    h.checksum = fnv1a(limbs, n_limbs);
     h.reserved = 0;
+    h.checksum = 0;
+    h.checksum = ckpt_checksum(h, limbs, n_limbs);
     std::memcpy(hdr, &h, sizeof h);
     std::string tmp = std::string(path) + ".tmp";
     FILE* f = std::fopen(tmp.c_str(), "wb");
     if (!f) return fail(PL_ERR_IO, "cannot open " + tmp);
+    // written, flushed and synced to the disk before it replaces the last
+    // good checkpoint (the order of P/checkpoint_manager.cpp:13-19)
     bool ok = std::fwrite(hdr, 1, 64, f) == 64 && std::fwrite(limbs, 8, n_limbs, f) == n_limbs;
     ok = ok && std::fflush(f) == 0;
-    std::fclose(f);
-    if (!ok) return fail(PL_ERR_IO, "short write to " + tmp);
-    if (std::rename(tmp.c_str(), path) != 0) return fail(PL_ERR_IO, "rename failed for " + tmp);
+    ok = ok && ::fsync(fileno(f)) == 0;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        std::remove(tmp.c_str());
+        return fail(PL_ERR_IO, "short write or sync failure on " + tmp);
+    }
+    if (std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        return fail(PL_ERR_IO, "rename failed for " + tmp);
+    }
+    // the rename itself reaches the disk with the directory's sync
+    std::string dir = path;
+    const size_t slash = dir.find_last_of('/');
+    dir = slash == std::string::npos ? "." : (slash == 0 ? "/" : dir.substr(0, slash));
+    const int dfd = ::open(dir.c_str(), O_RDONLY | O_DIRECTORY);
+    if (dfd >= 0) {
+        (void)::fsync(dfd);
+        ::close(dfd);
+    }
     return PL_OK;
 }
 
@@ -448,7 +481,8 @@ int pl_load_checkpoint(const char* path, uint64_t k, uint32_t n, uint64_t* base,
             std::vector<uint64_t> buf(h.n_limbs);
             std::fseek(f, 64, SEEK_SET);
             if (std::fread(buf.data(), 8, h.n_limbs, f) != h.n_limbs) rc = fail(PL_ERR_IO, "short read");
-            else if (fnv1a(buf.data(), buf.size()) != h.checksum) rc = fail(PL_ERR_IO, "checkpoint checksum mismatch");
+            else if (ckpt_checksum(h, buf.data(), buf.size()) != h.checksum)
+                rc = fail(PL_ERR_IO, "checkpoint checksum mismatch");
             else {
                 for (size_t i = 0; i < n_limbs; ++i) limbs[i] = i < buf.size() ? buf[i] : 0;
                 for (size_t i = n_limbs; i < buf.size(); ++i)

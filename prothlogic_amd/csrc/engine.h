@@ -116,16 +116,22 @@ class Group {
     // kernels per steady-state squaring: 2 (one-launch outer pass + row
     // pass), 3 (split outer pass in two launches + row pass) or 5 (unfused)
     int launches_per_squaring() const { return turn() ? 2 : (plan_.nt ? 3 : 5); }
-    // the one-launch outer pass is used (runs of 8 digits; with 16 its
+    // the one-launch outer pass can be used (runs of 8 digits; with 16 its
     // registers would spill)
     bool turn() const { return turn_ && plan_.nt == 8; }
     // Use the split outer pass even where the one-launch pass would run.  The
     // one-launch pass's block 0 waits on its slot's last block, which starts
     // after it: a launch only finishes when all its blocks can eventually be
-    // resident.  Kernels of other streams that spin the same way can hold
-    // the CUs that block needs, so callers that overlap groups on several
+    // resident.  Two such launches in flight at once can each hold the CUs
+    // the other's last blocks need, so callers that overlap groups on several
     // streams (the pool) take the split pass.  Call before the first step.
     void no_turn() { turn_ = false; }
+    // Chunks of plain squarings run the one-launch pass only while they hold
+    // their device's turn token (at most one holder per device in the
+    // process; others take the split pass for that chunk), and a chunk whose
+    // carry hand-off gave up is recomputed from its saved start on the split
+    // pass.  Number of such recomputations so far:
+    uint32_t handoff_fallbacks() const { return fallbacks_; }
 
     // Average duration (ms) of each kernel over all slots, timed with HIP
     // events between consecutive kernels of `reps` real plain squarings on
@@ -149,9 +155,12 @@ class Group {
   private:
     Params params(uint32_t slot0, bool check_active, uint32_t err_col) const;
     void launch(int kind, const Params& p, uint32_t slots);
-    void enqueue_plain_chunk(int g);
+    void enqueue_plain_chunk(int g, bool use_turn);
     void enqueue_step_chunk(int g);
-    void build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g);
+    void build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g, bool use_turn);
+    void run_plain_chunk(int g, bool use_turn);
+    void save_start(uint32_t slot0, uint32_t slots);
+    void restore_start(uint32_t slot0, uint32_t slots);
     double read_err(int g, double* trace, double* slot_err, uint32_t trace_slot = 0, uint32_t nslots = 0);
     void enqueue_err_clear(uint32_t slots);
     void enqueue_err_reduce(uint32_t slot0, uint32_t slots);
@@ -164,7 +173,11 @@ class Group {
     void* mem_ = nullptr;
     double2* z_ = nullptr;
     long long* carries_ = nullptr;
-    long long* runc_ = nullptr;
+    long long* runc_ = nullptr;   // split pass: plain carry-out triples of every run
+    long long* runt_ = nullptr;   // one-launch pass: tagged carry words (written by nothing else,
+                                  // so every word carries the last launch's tag)
+    double2* zbak_ = nullptr;     // start of the running chunk (one-launch pass fallback)
+    long long* xbak_ = nullptr;
     unsigned* epoch_ = nullptr;   // cap x 2: next and last carry hand-off epoch
     int* rdig_ = nullptr;
     long long* xhi_ = nullptr;
@@ -181,13 +194,15 @@ class Group {
     // scheduled-step chunks, one graph per launched slot count
     std::map<uint32_t, std::pair<hipGraphExec_t, hipGraph_t>> step_graphs_;
     uint32_t step_n_ = 0;  // slots the step chunk being enqueued launches
-    // plain squaring chunks, one graph per slot range (slot0 << 32 | slots)
+    // plain squaring chunks, one graph per slot range and outer-pass form
+    // (slot0 << 33 | slots << 1 | one-launch)
     std::map<uint64_t, std::pair<hipGraphExec_t, hipGraph_t>> plain_graphs_;
     uint32_t sq0_ = 0, sqn_ = 1;
     uint32_t pending_steps_ = 0;  // steps enqueued by step_begin, not yet read back
     bool uniform_ = true;  // every slot holds the plan's n and k
     bool turn_ = true;     // split outer pass as one launch (k_carry_turn); PL_CARRY_TURN=0: two
     bool give_up_ = false; // PL_HANDOFF_GIVE_UP=1: every carry hand-off wait gives up (tests)
+    uint32_t fallbacks_ = 0;
     std::vector<unsigned long long> err_host_;
 };
 

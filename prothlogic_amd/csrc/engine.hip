@@ -53,6 +53,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -1379,13 +1380,44 @@ static void host_slot_weights(const Plan& pl, uint64_t n, double2* wrf, double2*
 
 static uint32_t carry_tpb_for(const Plan& p) { return p.L >= p.run * 256u * 16u ? 256 : 64; }
 
+// The turn token of a device: held by at most one plain-squaring chunk at a
+// time in this process, for as long as its one-launch outer passes are in
+// flight (until the chunk has been read back).  Two one-launch passes in
+// flight at once could each hold the CUs the other's last blocks need; a
+// launch that has no spinning neighbour always finishes, because every other
+// kernel's blocks complete without waiting and free their CUs.  Kernels of
+// other processes are outside this token: their effect is bounded by the
+// waits' give-up and the chunk's recomputation on the split pass.
+namespace {
+constexpr int kMaxTokenDevices = 64;
+std::mutex g_turn_token[kMaxTokenDevices];
+
+struct TurnToken {
+    int dev = -1;
+    bool held = false;
+    TurnToken(int device, bool want)
+    {
+        if (want && device >= 0 && device < kMaxTokenDevices && g_turn_token[device].try_lock()) {
+            dev = device;
+            held = true;
+        }
+    }
+    ~TurnToken()
+    {
+        if (held) g_turn_token[dev].unlock();
+    }
+    TurnToken(const TurnToken&) = delete;
+    TurnToken& operator=(const TurnToken&) = delete;
+};
+}  // namespace
+
 size_t Group::slot_bytes(const Plan& p)
 {
     // residue (16 B/pt) + split-pass digits (4 B/pt) + run carries (24 B per
     // run) + flat-carry block carries + factored weights + scalars
     const size_t runs = p.nt ? p.L / p.nt : 0;
     const uint32_t tpb = carry_tpb_for(p);
-    return (size_t)p.L * 16 + (size_t)p.L * 4 + runs * 24 + (size_t)(p.L / (p.run * tpb)) * 24 +
+    return (size_t)p.L * 16 + (size_t)p.L * 4 + runs * (p.nt == 8 ? 48 : 24) + (size_t)(p.L / (p.run * tpb)) * 24 +
            (size_t)(p.R + p.C) * 32 + 64 + kChunk * 8;
 }
 
@@ -1408,6 +1440,7 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     const size_t o_z = off; off += al(sizeof(double2) * p.L * cap_);
     const size_t o_car = off; off += al(sizeof(long long) * 3 * carry_blocks_ * cap_);
     const size_t o_runc = off; off += al(sizeof(long long) * 3 * runs * cap_);
+    const size_t o_runt = off; off += al(p.nt == 8 ? sizeof(long long) * 3 * runs * cap_ : 0);
     const size_t o_epoch = off; off += al(sizeof(unsigned) * 2 * cap_);
     const size_t o_rdig = off; off += al(p.nt ? sizeof(int) * (size_t)p.L * cap_ : 0);
     const size_t o_xhi = off; off += al(sizeof(long long) * cap_);
@@ -1441,6 +1474,7 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     z_ = reinterpret_cast<double2*>(b + o_z);
     carries_ = reinterpret_cast<long long*>(b + o_car);
     runc_ = reinterpret_cast<long long*>(b + o_runc);
+    runt_ = p.nt == 8 ? reinterpret_cast<long long*>(b + o_runt) : nullptr;
     epoch_ = reinterpret_cast<unsigned*>(b + o_epoch);
     rdig_ = reinterpret_cast<int*>(b + o_rdig);
     xhi_ = reinterpret_cast<long long*>(b + o_xhi);
@@ -1503,6 +1537,8 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
 
 Group::~Group()
 {
+    // nothing launched on the stream may still use the graphs or the memory
+    if (stream_) (void)hipStreamSynchronize(stream_);
     for (auto& kv : plain_graphs_) {
         (void)hipGraphExecDestroy(kv.second.first);
         (void)hipGraphDestroy(kv.second.second);
@@ -1512,6 +1548,7 @@ Group::~Group()
         (void)hipGraphDestroy(kv.second.second);
     }
     if (mem_) (void)hipFree(mem_);
+    if (zbak_) (void)hipFree(zbak_);
     if (stream_) (void)hipStreamDestroy(stream_);
     delete tabs_;
 }
@@ -1659,15 +1696,17 @@ void Group::download(uint32_t slot, std::vector<int32_t>& digits, int64_t& xhi)
 // g squarings of every slot from the digit state back to the digit state;
 // iteration i's round-off lands in err column i.
 // g plain squarings of slots [sq0_, sq0_ + sqn_).
-void Group::enqueue_plain_chunk(int g)
+void Group::enqueue_plain_chunk(int g, bool use_turn)
 {
     const uint32_t s0 = sq0_, ns = sqn_;
     launch(kFwdCol, params(s0, false, 0), ns);
     launch(kRow, params(s0, false, 0), ns);
     for (int i = 0; i + 1 < g; ++i) {
         const Params p = params(s0, false, (uint32_t)i);
-        if (turn()) {
-            launch(kCarryTurn, p, ns);
+        if (use_turn) {
+            Params pt = p;
+            pt.runc = runt_;
+            launch(kCarryTurn, pt, ns);
         } else if (plan_.nt) {
             launch(kInvCarry, p, ns);
             launch(kAdjFwd, p, ns);
@@ -1686,13 +1725,13 @@ void Group::enqueue_plain_chunk(int g)
 }
 
 // g scheduled steps of every active slot; step i's round-off in err column i.
+// Step chunks take the split outer pass: the pool overlaps them on several
+// streams, and a single test's seed powering is a few dozen steps.
 void Group::enqueue_step_chunk(int g)
 {
     for (int i = 0; i < g; ++i) {
         const Params p = params(0, true, (uint32_t)i);
-        if (turn()) {
-            launch(kCarryTurn, p, step_n_);
-        } else if (plan_.nt) {
+        if (plan_.nt) {
             launch(kInvCarry, p, step_n_);
             launch(kAdjFwd, p, step_n_);
         } else {
@@ -1706,13 +1745,13 @@ void Group::enqueue_step_chunk(int g)
     }
 }
 
-void Group::build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g)
+void Group::build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g, bool use_turn)
 {
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     const uint32_t ns = stepping ? step_n_ : cap_;
     enqueue_err_clear(ns);
     if (stepping) enqueue_step_chunk(g);
-    else enqueue_plain_chunk(g);
+    else enqueue_plain_chunk(g, use_turn);
     enqueue_err_reduce(0, ns);
     check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
     check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -1757,6 +1796,54 @@ double Group::read_err(int g, double* trace, double* slot_err, uint32_t trace_sl
 
 double Group::square(uint64_t iters, double* err_trace) { return square_slots(0, cap_, iters, err_trace); }
 
+// One plain chunk of g squarings of slots [sq0_, sq0_ + sqn_): a captured
+// graph for whole chunks, direct launches for a partial one.
+void Group::run_plain_chunk(int g, bool use_turn)
+{
+    if (g == kChunk) {
+        const uint64_t key = ((uint64_t)sq0_ << 33) | ((uint64_t)sqn_ << 1) | (use_turn ? 1u : 0u);
+        auto it = plain_graphs_.find(key);
+        if (it == plain_graphs_.end()) {
+            hipGraphExec_t ex = nullptr;
+            hipGraph_t gr = nullptr;
+            build_graph(ex, gr, false, kChunk, use_turn);
+            it = plain_graphs_.emplace(key, std::make_pair(ex, gr)).first;
+        }
+        check(hipGraphLaunch(it->second.first, stream_), "hipGraphLaunch");
+    } else {
+        enqueue_err_clear(cap_);
+        enqueue_plain_chunk(g, use_turn);
+        enqueue_err_reduce(0, cap_);
+        check(hipGetLastError(), "square launch");
+    }
+}
+
+// The digit state of slots [slot0, slot0 + slots) before a one-launch chunk
+// (device-to-device; the backup is allocated on first use).
+void Group::save_start(uint32_t slot0, uint32_t slots)
+{
+    if (!zbak_) {
+        void* m = nullptr;
+        check(hipMalloc(&m, sizeof(double2) * plan_.L * cap_ + 256 + sizeof(long long) * cap_), "hipMalloc backup");
+        zbak_ = static_cast<double2*>(m);
+        xbak_ = reinterpret_cast<long long*>(static_cast<char*>(m) + sizeof(double2) * plan_.L * cap_ + 256);
+    }
+    check(hipMemcpyAsync(zbak_, z_ + (size_t)slot0 * plan_.L, sizeof(double2) * plan_.L * slots,
+                         hipMemcpyDeviceToDevice, stream_),
+          "backup z");
+    check(hipMemcpyAsync(xbak_, xhi_ + slot0, sizeof(long long) * slots, hipMemcpyDeviceToDevice, stream_),
+          "backup xhi");
+}
+
+void Group::restore_start(uint32_t slot0, uint32_t slots)
+{
+    check(hipMemcpyAsync(z_ + (size_t)slot0 * plan_.L, zbak_, sizeof(double2) * plan_.L * slots,
+                         hipMemcpyDeviceToDevice, stream_),
+          "restore z");
+    check(hipMemcpyAsync(xhi_ + slot0, xbak_, sizeof(long long) * slots, hipMemcpyDeviceToDevice, stream_),
+          "restore xhi");
+}
+
 double Group::square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, double* err_trace)
 {
     if (slots == 0 || slot0 + slots > cap_) throw std::invalid_argument("square_slots: slot range out of bounds");
@@ -1767,23 +1854,30 @@ double Group::square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, doubl
     uint64_t done = 0;
     while (done < iters) {
         const int g = (int)std::min<uint64_t>(kChunk, iters - done);
-        if (g == kChunk) {
-            const uint64_t key = ((uint64_t)slot0 << 32) | slots;
-            auto it = plain_graphs_.find(key);
-            if (it == plain_graphs_.end()) {
-                hipGraphExec_t ex = nullptr;
-                hipGraph_t gr = nullptr;
-                build_graph(ex, gr, false, kChunk);
-                it = plain_graphs_.emplace(key, std::make_pair(ex, gr)).first;
+        double* tr = err_trace ? err_trace + done : nullptr;
+        // one-launch passes need this device's turn token and g >= 2 (a
+        // chunk of one squaring has no steady-state squaring)
+        TurnToken token(device_, turn() && g >= 2);
+        if (token.held) {
+            save_start(slot0, slots);
+            run_plain_chunk(g, true);
+            double e = read_err(g, tr, nullptr, slot0);
+            const int st = status();
+            if (st & 2) {
+                // a hand-off wait gave up (another process's kernels held the
+                // CUs its predecessor needed, or the test hook): the chunk's
+                // result is void; recompute it from its start on the split pass
+                clear_status();
+                restore_start(slot0, slots);
+                run_plain_chunk(g, false);
+                e = read_err(g, tr, nullptr, slot0);
+                ++fallbacks_;
             }
-            check(hipGraphLaunch(it->second.first, stream_), "hipGraphLaunch");
+            emax = std::max(emax, e);
         } else {
-            enqueue_err_clear(cap_);
-            enqueue_plain_chunk(g);
-            enqueue_err_reduce(0, cap_);
-            check(hipGetLastError(), "square launch");
+            run_plain_chunk(g, false);
+            emax = std::max(emax, read_err(g, tr, nullptr, slot0));
         }
-        emax = std::max(emax, read_err(g, err_trace ? err_trace + done : nullptr, nullptr, slot0));
         done += g;
     }
     return emax;
@@ -1856,7 +1950,7 @@ void Group::step_begin(uint32_t g, uint32_t slots)
         if (it == step_graphs_.end()) {
             hipGraphExec_t ex = nullptr;
             hipGraph_t gr = nullptr;
-            build_graph(ex, gr, true, kStepChunk);
+            build_graph(ex, gr, true, kStepChunk, false);
             it = step_graphs_.emplace(ns, std::make_pair(ex, gr)).first;
         }
         check(hipGraphLaunch(it->second.first, stream_), "hipGraphLaunch");
@@ -1944,8 +2038,11 @@ void Group::kernel_times(int reps, double* ms)
     // as it runs inside a squaring chunk.  The first and last kernels of a
     // chunk (forward column pass from digits, inverse column + carry + fix-up
     // back to digits) are timed on the same pass around the loop.
+    TurnToken token(device_, turn());
+    Params pt = p;
+    pt.runc = runt_;
     std::vector<int> seq;
-    if (turn())
+    if (token.held)
         seq = { kCarryTurn, kRow };
     else if (plan_.nt)
         seq = { kInvCarry, kAdjFwd, kRow };
@@ -1968,7 +2065,7 @@ void Group::kernel_times(int reps, double* ms)
     const size_t loop0 = e;
     for (int r = 0; r < reps; ++r)
         for (int j = 0; j < m; ++j) {
-            launch(seq[(size_t)j], p, cap_);
+            launch(seq[(size_t)j], seq[(size_t)j] == kCarryTurn ? pt : p, cap_);
             check(hipEventRecord(ev[++e], stream_), "hipEventRecord");
         }
     // leave: back to digits
@@ -2002,7 +2099,7 @@ void Group::kernel_times(int reps, double* ms)
         launch(kRow, p, cap_);
         check(hipEventRecord(a, stream_), "hipEventRecord");
         for (int r = 0; r < reps; ++r)
-            for (int j = 0; j < m; ++j) launch(seq[(size_t)j], p, cap_);
+            for (int j = 0; j < m; ++j) launch(seq[(size_t)j], seq[(size_t)j] == kCarryTurn ? pt : p, cap_);
         check(hipEventRecord(b, stream_), "hipEventRecord");
         launch(kInvCol, p, cap_);
         launch(kCarry, p, cap_);

@@ -33,7 +33,15 @@ struct Request {
     uint64_t tag;
     uint64_t stop_after;
     double t_submit;
+    uint32_t attempts;  // runs restarted after the device carry did not settle
 };
+
+// A test whose round-off passes this (as pl_proth_test's default guard) has
+// no reliable residue; its result is reported void.
+constexpr double kErrGuard = 0.40;
+// Restarts from the seed after a group's carry did not settle, before the
+// test is reported void.
+constexpr uint32_t kMaxAttempts = 2;
 
 struct Occupant {
     bool used = false;
@@ -152,6 +160,43 @@ void admit(pl_pool* P)
     }
 }
 
+pl_pool_result void_result(pl_pool* P, const Request& q, uint32_t L, int error, double max_err)
+{
+    pl_pool_result r{};
+    r.tag = q.tag;
+    r.k = q.k;
+    r.n = q.n;
+    r.fft_len = L;
+    r.verdict = -2;
+    r.error = error;
+    r.max_err = max_err;
+    r.base = pl_choose_base(q.k, q.n);
+    r.t_submit = q.t_submit;
+    r.t_done = P->now();
+    return r;
+}
+
+// Takes a slot's test off the device after its group's carry did not settle:
+// the test starts again from its seed (ahead of later arrivals), or is
+// reported void once it has been restarted kMaxAttempts times.
+void evict(pl_pool* P, LenGroup& lg, uint32_t s)
+{
+    Occupant& o = lg.slots[s];
+    Request q = o.req;
+    lg.g->set_state(s, 0, false);
+    if (++q.attempts > kMaxAttempts) P->done.push_back(void_result(P, q, lg.plan.L, PL_ERR_STATE, o.max_err));
+    else P->queue.push_front(q);
+    o.used = false;
+    o.M.reset();
+}
+
+void evict_group(pl_pool* P, LenGroup& lg)
+{
+    lg.g->clear_status();
+    for (uint32_t s = 0; s < lg.slots.size(); ++s)
+        if (lg.slots[s].used) evict(P, lg, s);
+}
+
 void retire(pl_pool* P)
 {
     for (auto& kv : P->groups) {
@@ -161,6 +206,20 @@ void retire(pl_pool* P)
             Occupant& o = lg.slots[s];
             if (!o.used || st[s].active) continue;
             o.max_err = std::max(o.max_err, lg.g->leave((uint32_t)s));
+            if (lg.g->status()) {
+                // the last squaring's carry did not settle: no reliable residue
+                lg.g->clear_status();
+                evict(P, lg, (uint32_t)s);
+                continue;
+            }
+            if (!(o.max_err <= kErrGuard)) {
+                P->done.push_back(void_result(P, o.req, lg.plan.L, PL_ERR_ROUNDOFF, o.max_err));
+                P->done.back().t_admit = o.t_admit;
+                P->done.back().squarings = o.total;
+                o.used = false;
+                o.M.reset();
+                continue;
+            }
             std::vector<int32_t> d;
             int64_t xhi = 0;
             lg.g->download((uint32_t)s, d, xhi);
@@ -177,6 +236,7 @@ void retire(pl_pool* P)
             const bool full = !o.req.stop_after || o.req.stop_after >= (uint64_t)o.req.n - 1;
             r.verdict = full ? (plbn::is_n_minus_1(*o.M, x) ? 1 : 0) : -1;
             r.gated = 0;
+            r.error = PL_OK;
             r.t_submit = o.req.t_submit;
             r.t_admit = o.t_admit;
             r.t_done = P->now();
@@ -235,11 +295,12 @@ int pl_pool_submit(pl_pool* P, uint64_t k, uint32_t n, uint64_t tag, uint64_t st
         r.n = n;
         r.gated = 1;
         r.verdict = 0;
+        r.error = PL_OK;
         r.t_submit = r.t_admit = r.t_done = t;
         P->done.push_back(r);
         return PL_OK;
     }
-    P->queue.push_back(Request{ k, n, tag, stop_after, t });
+    P->queue.push_back(Request{ k, n, tag, stop_after, t, 0 });
     return PL_OK;
 }
 
@@ -275,14 +336,10 @@ int pl_pool_run(pl_pool* P, uint32_t steps, pl_pool_result* out, size_t cap, siz
             }
             done += g;
         }
-        for (LenGroup* lg : live) {
-            const int st = lg->g->status();
-            if (st) {
-                lg->g->clear_status();
-                return pfail(PL_ERR_STATE, st & 2 ? "carry hand-off between workgroups timed out"
-                                                  : "device carry did not settle");
-            }
-        }
+        // A group whose carry did not settle in these steps holds no reliable
+        // residue in any slot: its tests restart from their seeds.
+        for (LenGroup* lg : live)
+            if (lg->g->status()) evict_group(P, *lg);
         retire(P);
         admit(P);
     } catch (const std::exception& e) {

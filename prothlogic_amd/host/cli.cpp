@@ -102,6 +102,12 @@ size_t test_all(const std::vector<uint64_t>& ks, unsigned n, const std::string& 
             const pl_pool_result& r = res[i];
             --pending;
             ++done;
+            if (r.error != PL_OK) {
+                // void result (round-off above the guard, or carries that never settled)
+                std::cout << "\n[ERROR] " << r.k << " * 2^" << r.n << " + 1: no verdict (error " << r.error
+                          << ", max err " << r.max_err << ")" << std::endl;
+                continue;
+            }
             if (r.verdict == 1) {
                 ++primes;
                 std::cout << "\n[!] FOUND PRIME: " << r.k << " * 2^" << r.n << " + 1 (a=" << r.base << ")" << std::endl;

@@ -3,12 +3,15 @@
 // callers do, and prints one JSON line per case for an external checker.
 // Run it in an empty directory: it writes checkpoints and primes.txt there.
 //
-//   prothhost_selftest
+//   prothhost_selftest                    the mirror's call sites, one by one
+//   prothhost_selftest threads k:n,k:n    one full test per thread, at once
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <utility>
 #include <vector>
 
 #include "autotune.h"
@@ -127,10 +130,61 @@ void worker_tasks(unsigned n, const std::string& ks, uint64_t single_k)
     fft_destroy(ctx);
 }
 
+// One execute_proth_test_fft per std::thread, each with its own FFTContext
+// and limbs, as the reference's dispatch runs its workers (P/dcore.cpp:134-175).
+// spec: "k:n,k:n,...".
+int worker_threads(const std::string& spec)
+{
+    std::vector<std::pair<uint64_t, unsigned>> cands;
+    size_t pos = 0;
+    while (pos < spec.size()) {
+        size_t comma = spec.find(',', pos);
+        if (comma == std::string::npos) comma = spec.size();
+        const std::string item = spec.substr(pos, comma - pos);
+        const size_t colon = item.find(':');
+        if (colon == std::string::npos) return 2;
+        cands.emplace_back(std::stoull(item.substr(0, colon)), (unsigned)std::stoul(item.substr(colon + 1)));
+        pos = comma + 1;
+    }
+    std::vector<std::string> lines(cands.size());
+    std::vector<std::thread> ts;
+    for (size_t i = 0; i < cands.size(); ++i)
+        ts.emplace_back([&, i] {
+            const uint64_t k = cands[i].first;
+            const unsigned n = cands[i].second;
+            char buf[256];
+            try {
+                uint32_t L = 0;
+                if (pl_plan_fft_len(n, k, &L) != PL_OK) throw std::runtime_error(pl_last_error());
+                FFTContext ctx;
+                if (!fft_init(ctx, FFTBackend::INTERNAL, L)) throw std::runtime_error("fft_init failed");
+                std::vector<uint64_t> limbs(limbs_for(L, n), 0);
+                const bool prime = execute_proth_test_fft(k, n, limbs.data(), ctx, 0);
+                std::snprintf(buf, sizeof buf,
+                              "{\"case\": \"thread\", \"k\": %llu, \"n\": %u, \"prime\": %d, \"res64\": \"%016llx\"}",
+                              (unsigned long long)k, n, prime ? 1 : 0, (unsigned long long)limbs[0]);
+                fft_destroy(ctx);
+            } catch (const std::exception& e) {
+                std::snprintf(buf, sizeof buf, "{\"case\": \"error\", \"k\": %llu, \"what\": \"%s\"}",
+                              (unsigned long long)k, e.what());
+            }
+            lines[i] = buf;
+        });
+    for (auto& t : ts) t.join();
+    int rc = 0;
+    for (const auto& l : lines) {
+        std::printf("%s\n", l.c_str());
+        if (l.find("\"error\"") != std::string::npos) rc = 1;
+    }
+    std::fflush(stdout);
+    return rc;
+}
+
 }  // namespace
 
-int main()
+int main(int argc, char** argv)
 {
+    if (argc >= 3 && std::string(argv[1]) == "threads") return worker_threads(argv[2]);
     try {
         std::printf("{\"case\": \"start\", \"abi\": %d}\n", pl_abi_version());
         std::fflush(stdout);

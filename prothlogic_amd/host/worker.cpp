@@ -46,6 +46,11 @@ std::string batch(const std::vector<uint64_t>& ks, unsigned n, int device)
             for (size_t i = 0; i < got; ++i) {
                 const pl_pool_result& r = res[i];
                 --pending;
+                if (r.error != PL_OK) {
+                    std::lock_guard<std::mutex> lock(g_batch_mutex);
+                    std::cout << "[BATCH-RESULT] k=" << r.k << ": no verdict (error " << r.error << ")\n";
+                    continue;
+                }
                 if (r.verdict != 1) continue;
                 std::lock_guard<std::mutex> lock(g_batch_mutex);
                 std::cout << "[BATCH-RESULT] PRIME FOUND: k=" << r.k << "\n";

@@ -12,6 +12,7 @@ with a chosen by the reference's base rule (P/proth_logic.cpp:423-435).
   python tests/golden/make_golden.py kats      # ~5 min (n=120002 KATs)
   python tests/golden/make_golden.py large     # ~10-20 min (n=2^21, 2^24)
   python tests/golden/make_golden.py batch     # c4 subset, ~10 min
+  python tests/golden/make_golden.py c5        # c5 subset, ~5 min
 """
 from __future__ import annotations
 
@@ -94,6 +95,14 @@ def large():
     write("large_cases", {"generator": "tests/golden/make_golden.py large", "cases": cases})
 
 
+def serving():
+    # the first 32 candidates of the serving workload (config 5) after 200
+    # squarings past the seed; their n span 5e4..4e6
+    w = W.c5(32)
+    cases = [case(k, n, [200]) for (k, n) in w.candidates]
+    write("c5_cases", {"generator": "tests/golden/make_golden.py c5", "workload": w.name, "cases": cases})
+
+
 def batch():
     w = W.c4()
     cases = [case(k, n, [200]) for (k, n) in w.candidates[:32]]
@@ -104,4 +113,4 @@ def batch():
 
 if __name__ == "__main__":
     for what in sys.argv[1:] or ["small"]:
-        {"small": small, "kats": kats, "large": large, "batch": batch}[what]()
+        {"small": small, "kats": kats, "large": large, "batch": batch, "c5": serving}[what]()

@@ -79,6 +79,36 @@ def test_checkpoint_v2_roundtrip_and_corruption(tmp_path):
         core.load_checkpoint(p, k, n, 1900)              # truncated
 
 
+def test_checkpoint_header_is_covered_by_the_checksum(tmp_path):
+    """A flipped next_iter, base or n_limbs field in the header must be caught
+    like a flipped limb (the checksum covers header and limbs)."""
+    import struct
+    p = str(tmp_path / "ck.bin")
+    k, n = 843, 120002
+    x = (1 << 90000) + 999
+    core.save_checkpoint(p, k, n, 5, 30001, x, 1900)
+    good = Path(p).read_bytes()
+    assert good[:8] == b"PLCKPT02" and len(good) == 64 + 8 * 1900
+    # header fields: magic[8] k@8 n@16 base@20 next_iter@24 n_limbs@32 checksum@40
+    for off, fmt in ((24, "<Q"), (20, "<I")):
+        raw = bytearray(good)
+        (v,) = struct.unpack_from(fmt, raw, off)
+        struct.pack_into(fmt, raw, off, v ^ 1)
+        Path(p).write_bytes(bytes(raw))
+        with pytest.raises(core.PlError) as ei:
+            core.load_checkpoint(p, k, n, 1900)
+        assert ei.value.code == core.PL_ERR_IO
+    Path(p).write_bytes(good)
+    assert core.load_checkpoint(p, k, n, 1900) == (5, 30001, x)
+    assert not Path(p + ".tmp").exists()
+
+
+def test_checkpoint_write_failure_is_reported(tmp_path):
+    with pytest.raises(core.PlError) as ei:
+        core.save_checkpoint(str(tmp_path / "missing_dir" / "ck.bin"), 843, 120002, 5, 1, 7, 4)
+    assert ei.value.code == core.PL_ERR_IO
+
+
 def test_reads_reference_checkpoint_layout(tmp_path):
     from oracle import Oracle, int_to_limbs
     o = Oracle()

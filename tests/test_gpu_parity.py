@@ -141,8 +141,16 @@ def test_known_primes_full_test(gpu):
     for k, n in [(3, 20909), (3, 34350)]:
         r = gpu.proth_test(k, n)
         assert r["verdict"] == 1 and r["res64"] == 0 and r["base"] == 5
-    r = gpu.proth_test(3, 34351)   # composite (5 | N is not a gate hit: the gate covers p | N)
-    assert r["verdict"] == 0
+    # 5 divides 3*2^34351+1: the small-prime gate rejects it before any squaring
+    r = gpu.proth_test(3, 34351)
+    assert r["gated"] and r["verdict"] == 0 and r["iters_done"] == 0
+    # composites that pass the gate reach the verdict through the squarings
+    for k, n in [(1023, 32768), (3, 34353)]:
+        assert not gpu.gate_rejects(k, n)
+        prime, r64, a = G.proth_verdict(k, n)
+        r = gpu.proth_test(k, n)
+        assert not prime and r["verdict"] == 0 and not r["gated"]
+        assert r["res64"] == r64 and r["base"] == a and r["iters_done"] == n - 1
 
 
 def test_fft_square_karatsuba_shim_matches_oracle(gpu):
