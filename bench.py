@@ -4,12 +4,12 @@ one MI355X, with the residue resident in HBM.
 Workload (default): the n = 2^21 single large test -- k drawn odd in
 [3, 2^20) from PCG64(seed=2) (gate survivors only), base a by the reference's
 rule, seed a^k mod N built on the device.  A "step" is one squaring
-x <- x^2 mod N.  W untimed warm-up squarings run on the same context (the
-default, 256, is one whole chunk, so the engine has captured its chunk graph
-before the clock starts); then the residue is reset to the seed and exactly
-K squarings are timed with the device synchronised on both sides.  Res64
-after 10^4 squarings from the seed (an untimed second run) is checked
-against the GMP fixture.
+x <- x^2 mod N.  W untimed warm-up squarings run on the same context from
+the seed (the default, 256, is one whole chunk, so the engine has captured
+its chunk graph before the clock starts); then exactly the next K squarings
+are timed with the device synchronised on both sides.  Res64 after 10^4
+squarings from the seed (an untimed second run) is checked against the GMP
+fixture.
 
 The same line reports the n = 2^24 test (1024 squarings timed, Res64 checked
 after 10^3), the per-kernel roofline (HIP events on the engine's stream), and
@@ -61,15 +61,16 @@ def seed_context(core, k, n):
 
 
 def time_squarings(core, k, n, steps, warmup, verify_at=None):
-    """Times `steps` squarings from the seed a^k; Res64 is read after them, or,
-    with verify_at, after that many squarings from the seed in a second,
-    untimed run (so the timed count can be whole graph chunks while the
-    check uses a count the GMP fixtures hold)."""
+    """Times `steps` squarings that follow `warmup` untimed ones from the
+    seed a^k (the warm-up takes the test past its first squarings, whose
+    exact values are still small and run in normal form, and captures the
+    chunk graph); Res64 is read after them, or, with verify_at, after that
+    many squarings from the seed in a second, untimed run (a count the GMP
+    fixtures hold)."""
     c, a = seed_context(core, k, n)
     seed = c.get_residue()
     if warmup:
         c.square(warmup)
-    c.set_residue(seed)
     c.square(0)
     t0 = time.perf_counter()
     emax = c.square(steps)

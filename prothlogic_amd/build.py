@@ -29,7 +29,8 @@ ARCH = os.environ.get("PL_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
              "-Xclang", "-packed-fp32-ops", "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 
-CORE_SRCS = [CSRC / "engine.hip", CSRC / "capi.cpp", CSRC / "bignum.cpp", CSRC / "pool.cpp", CSRC / "sieve.hip"]
+CORE_SRCS = [CSRC / "engine.hip", CSRC / "capi.cpp", CSRC / "bignum.cpp", CSRC / "prewrap.cpp", CSRC / "pool.cpp",
+             CSRC / "sieve.hip"]
 HOST_SRCS = [HOST / "fft_engine.cpp", HOST / "proth_logic.cpp", HOST / "checkpoint_manager.cpp",
              HOST / "worker.cpp", HOST / "autotune.cpp", HOST / "presieve_core.cpp"]
 

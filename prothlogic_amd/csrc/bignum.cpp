@@ -241,6 +241,39 @@ Limbs from_digits(const Modulus& M, uint32_t L, const std::vector<int32_t>& digi
     return out;
 }
 
+Limbs mul_kinv(const Modulus& M, const Limbs& x)
+{
+    Limbs y = mod_N(M, x);
+    Limbs t = mul_small_mod(M, mul_2n(M, y), M.k - 1);  // x*(k-1)*2^n
+    t.resize(M.nl + 1, 0);
+    y.resize(M.nl + 1, 0);
+    add_in(t, y);
+    return mod_N(M, t);
+}
+
+Limbs kinv_pow(const Modulus& M, unsigned e)
+{
+    Limbs r(M.nl, 0);
+    r[0] = 1;
+    for (unsigned i = 0; i < e; ++i) r = mul_kinv(M, r);
+    return r;
+}
+
+static uint64_t bitlen_limbs(const Limbs& x)
+{
+    for (size_t i = x.size(); i-- > 0;)
+        if (x[i]) return 64 * (uint64_t)i + bitlen_u64(x[i]);
+    return 0;
+}
+
+uint64_t distance_bits(const Modulus& M, const Limbs& x)
+{
+    Limbs y = mod_N(M, x);
+    Limbs d = M.N;
+    sub_in(d, y);  // N - y > 0
+    return std::min(bitlen_limbs(y), bitlen_limbs(d));
+}
+
 uint64_t low64(const Limbs& x) { return x.empty() ? 0 : x[0]; }
 
 bool is_n_minus_1(const Modulus& M, const Limbs& x)

@@ -42,6 +42,14 @@ void to_digits(const Modulus& M, uint32_t L, const Limbs& x, std::vector<int32_t
 // from_digits: sum digits[m]*2^e_m + xhi*2^n, reduced to [0, N).
 Limbs from_digits(const Modulus& M, uint32_t L, const std::vector<int32_t>& digits, int64_t xhi);
 
+// x*k^-1 mod N for x < N (k^-1 == (k-1)*2^n + 1, so one shift-fold and a
+// small multiply), and k^-e mod N.
+Limbs mul_kinv(const Modulus& M, const Limbs& x);
+Limbs kinv_pow(const Modulus& M, unsigned e);
+// Bit length of min(x, N - x) for x < N: how far a residue is from the
+// small values whose digits are sparse.
+uint64_t distance_bits(const Modulus& M, const Limbs& x);
+
 uint64_t low64(const Limbs& x);
 bool is_n_minus_1(const Modulus& M, const Limbs& x);
 unsigned bitlen_u64(uint64_t x);

@@ -16,6 +16,7 @@
 
 #include "bignum.h"
 #include "engine.h"
+#include "prewrap.h"
 
 namespace {
 
@@ -95,32 +96,85 @@ int jacobi_small(uint64_t b, uint64_t k, uint32_t n)
 
 struct pl_ctx {
     pleng::Plan plan;
-    std::unique_ptr<pleng::Group> eng;  // one slot
+    // slot 0: the residue; slots 1 .. extra: the caller's (Gerbicz check);
+    // then the normal-form scratch (a temporary and two constants)
+    std::unique_ptr<pleng::Group> eng;
     std::unique_ptr<plbn::Modulus> M;
     int device = 0;
+    plprep::Scratch scr;
+    uint64_t consts_a = ~0ull;  // the a whose constants the scratch holds (~0: none loaded)
+    // While the exact value is small, slot 0 holds x itself (normal form).
+    bool normal = false;
+    plprep::SmallValue small;
 
-    plbn::Limbs mont_value()  // canonical Montgomery form Y of the device residue
+    plbn::Limbs raw_of(uint32_t slot) { return plprep::raw(*eng, *M, slot); }
+    // Montgomery form of slot 0 (only outside the normal form)
+    plbn::Limbs mont_value() { return raw_of(0); }
+    void set_mont_value(const plbn::Limbs& y) { plprep::set_raw(*eng, *M, 0, y); }
+    plbn::Limbs mont_of(uint32_t slot) { return raw_of(slot); }
+    plbn::Limbs value()
     {
-        std::vector<int32_t> d;
-        int64_t xhi = 0;
-        eng->download(0, d, xhi);
-        return plbn::from_digits(*M, plan.L, d, xhi);
+        plbn::Limbs r = raw_of(0);
+        return normal ? r : plbn::from_mont(*M, r);
     }
-    void set_mont_value(const plbn::Limbs& y)
+    void set_value(const plbn::Limbs& x0)
     {
-        std::vector<int32_t> d;
-        int64_t xhi = 0;
-        plbn::to_digits(*M, plan.L, y, d, xhi);
-        eng->upload(0, d, xhi);
+        const plbn::Limbs x = plbn::mod_N(*M, x0);
+        normal = small.set(*M, x);
+        if (normal) plprep::set_raw(*eng, *M, 0, x);
+        else set_mont_value(plbn::to_mont(*M, x));
     }
-    plbn::Limbs value() { return plbn::from_mont(*M, mont_value()); }
-    void set_value(const plbn::Limbs& x) { set_mont_value(plbn::to_mont(*M, x)); }
-    plbn::Limbs mont_of(uint32_t slot)  // canonical Montgomery form of any slot
+    void ensure_consts(uint64_t a)
     {
-        std::vector<int32_t> d;
-        int64_t xhi = 0;
-        eng->download(slot, d, xhi);
-        return plbn::from_digits(*M, plan.L, d, xhi);
+        if (consts_a == ~0ull || (a && consts_a != a)) {
+            plprep::load_constants(*eng, *M, a, scr);
+            consts_a = a;
+        }
+    }
+    // One squaring in normal form: x <- x^2 * (times_a ? a : 1).
+    double normal_step(bool times_a, uint64_t a)
+    {
+        ensure_consts(times_a ? a : 0);
+        const double e = plprep::step(*eng, 0, scr, times_a);
+        small.grow(true, times_a ? a : 0);
+        return e;
+    }
+    // Normal form -> Montgomery form (once the exact value has wrapped).
+    double leave_normal()
+    {
+        ensure_consts(0);
+        const double e = plprep::to_mont(*eng, 0, scr);
+        normal = false;
+        return e;
+    }
+    bool wrapped() const { return small.wrapped(*M); }
+    // `iters` squarings of slot 0 (in normal form first while the value is small).
+    double square(uint64_t iters, double* trace)
+    {
+        double e = 0.0;
+        uint64_t done = 0;
+        while (normal && done < iters) {
+            if (wrapped()) break;
+            const double s = normal_step(false, 0);
+            if (trace) trace[done] = s;
+            e = std::max(e, s);
+            ++done;
+        }
+        if (normal && wrapped()) e = std::max(e, leave_normal());
+        if (done < iters) e = std::max(e, eng->square_slots(0, 1, iters - done, trace ? trace + done : nullptr));
+        return e;
+    }
+    // The normal-form squarings until the value has wrapped (at most `limit`),
+    // then the move to the Montgomery form; returns the squarings applied.
+    uint64_t finish_normal(uint64_t limit, double& emax)
+    {
+        uint64_t done = 0;
+        while (normal && !wrapped() && done < limit) {
+            emax = std::max(emax, normal_step(false, 0));
+            ++done;
+        }
+        if (normal && wrapped()) emax = std::max(emax, leave_normal());
+        return done;
     }
 };
 
@@ -142,7 +196,8 @@ int create_ctx(uint64_t k, uint32_t n, uint32_t fft_len, int device, uint32_t ca
         const char* fz = std::getenv("PL_FUSED");
         const bool allow_split = !(fz && fz[0] == '0');
         if (!pleng::choose_layout(c->plan, (double)n / c->plan.L, k, 1, allow_split, why)) return fail(PL_ERR_ARG, why);
-        c->eng = std::make_unique<pleng::Group>(c->plan, capacity, device);
+        c->eng = std::make_unique<pleng::Group>(c->plan, capacity + 3, device);
+        c->scr = plprep::Scratch{ capacity, capacity + 1, capacity + 2 };
         *out = c.release();
         return PL_OK;
     });
@@ -226,7 +281,7 @@ int pl_square(pl_ctx* ctx, uint64_t iters, double err_guard, double* max_err, do
     if (!ctx) return fail(PL_ERR_ARG, "NULL context");
     if (err_guard <= 0) err_guard = 0.40;
     return guarded([&] {
-        double e = ctx->eng->square(iters, err_trace);
+        double e = ctx->square(iters, err_trace);
         if (max_err) *max_err = e;
         int st = ctx->eng->status();
         if (st) ctx->eng->clear_status();
@@ -242,7 +297,9 @@ int pl_mul_small(pl_ctx* ctx, uint64_t a)
 {
     if (!ctx) return fail(PL_ERR_ARG, "NULL context");
     return guarded([&] {
-        ctx->set_mont_value(plbn::mul_small_mod(*ctx->M, ctx->mont_value(), a));
+        // linear in the raw state, so the same in either form
+        plprep::set_raw(*ctx->eng, *ctx->M, 0, plbn::mul_small_mod(*ctx->M, ctx->raw_of(0), a));
+        if (ctx->normal) ctx->small.grow(false, a);
         return PL_OK;
     });
 }
@@ -283,7 +340,7 @@ int pl_kernel_times(pl_ctx* ctx, int reps, double* ms, uint64_t* traffic_bytes)
 {
     if (!ctx || !ms || reps <= 0) return fail(PL_ERR_ARG, "bad argument");
     return guarded([&] {
-        ctx->eng->kernel_times(reps, ms);
+        ctx->eng->kernel_times(reps, ms, 1);
         if (traffic_bytes) *traffic_bytes = 32ull * ctx->plan.L;
         return PL_OK;
     });
@@ -335,7 +392,7 @@ int pl_autotune_fft_len(uint32_t n, uint64_t k, int device, double err_guard, ui
             pleng::check(hipEventCreate(&t0), "hipEventCreate");
             pleng::check(hipEventCreate(&t1), "hipEventCreate");
             pleng::check(hipEventRecord(t0, c->eng->stream()), "hipEventRecord");
-            c->eng->square(reps, nullptr);
+            c->eng->square_slots(0, 1, reps, nullptr);
             pleng::check(hipEventRecord(t1, c->eng->stream()), "hipEventRecord");
             pleng::check(hipEventSynchronize(t1), "hipEventSynchronize");
             pleng::check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
@@ -567,23 +624,18 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
             }
         }
         if (!resumed) {
-            // seed a^k mod N on the device: x = a, then one scheduled squaring
-            // per lower bit of k, multiplied by a where the bit is set
+            // seed a^k mod N on the device, left to right from x = a: one
+            // squaring per lower bit of k, times a where the bit is set, in
+            // normal form (the values are small)
             if (a == 0) {
                 ctx->set_value(plbn::Limbs(1, 0));
             } else {
                 ctx->set_value(plbn::Limbs(1, a));
                 unsigned top = 0;
                 while (top < 63 && (k >> (top + 1))) ++top;
-                if (top > 0) {
-                    ctx->eng->set_slot(0, n, k, (uint32_t)a, top);
-                    ctx->eng->set_state(0, 0, top > 1);
-                    ctx->eng->enter(0);
-                    double e = top > 1 ? ctx->eng->step(top - 1, nullptr) : 0.0;
-                    e = std::max(e, ctx->eng->leave(0));
-                    out->max_err = std::max(out->max_err, e);
-                    if (ctx->eng->status()) return fail(PL_ERR_STATE, "device carry did not settle (seed)");
-                }
+                for (int b = (int)top - 1; b >= 0; --b)
+                    out->max_err = std::max(out->max_err, ctx->normal_step(((k >> b) & 1) != 0, a));
+                if (ctx->eng->status()) return fail(PL_ERR_STATE, "device carry did not settle (seed)");
             }
         }
         out->resumed_at = next;
@@ -602,11 +654,11 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
         bool injected = false;
         auto maybe_inject = [&] {
             if (!args->inject_error_at || injected || next <= args->inject_error_at) return;
-            plbn::Limbs y = ctx->mont_value();
+            plbn::Limbs y = ctx->raw_of(0);
             plbn::Limbs one(y.size(), 0);
             one[0] = 1;
             plbn::add_in(y, one);
-            ctx->set_mont_value(plbn::mod_N(*ctx->M, y));
+            plprep::set_raw(*ctx->eng, *ctx->M, 0, plbn::mod_N(*ctx->M, y));
             injected = true;
         };
         if (gerbicz) {
@@ -614,6 +666,23 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
             const uint64_t B = args->gerbicz_block;
             const uint64_t Mb = args->gerbicz_blocks ? args->gerbicz_blocks : 64;
             const uint64_t end = args->stop_after ? std::min<uint64_t>(total, next + args->stop_after) : total;
+            // the epochs run on Montgomery forms: first the squarings that
+            // still have a small exact value (a few dozen at most)
+            {
+                double e = 0.0;
+                const uint64_t d = ctx->finish_normal(end - next, e);
+                out->max_err = std::max(out->max_err, e);
+                if (G.status()) return fail(PL_ERR_STATE, "device carry did not settle (normal form)");
+                next += d;
+                out->iters_done += d;
+                if (ctx->normal && next < end) {
+                    // the value stays small (0 or 1): the squarings go on in normal form
+                    e = ctx->square(end - next, nullptr);
+                    out->max_err = std::max(out->max_err, e);
+                    out->iters_done += end - next;
+                    next = end;
+                }
+            }
             uint64_t verified = next, last_ckpt = next;
             int failures = 0;
             auto start_epoch = [&] {
@@ -679,7 +748,7 @@ int pl_proth_test(const pl_test_args* args, pl_test_result* out)
             if (args->stop_after && out->iters_done + chunk > args->stop_after)
                 chunk = args->stop_after - out->iters_done;
             if (chunk == 0) break;
-            double e = ctx->eng->square(chunk, nullptr);
+            double e = ctx->square(chunk, nullptr);
             if (e > out->max_err) out->max_err = e;
             int st = ctx->eng->status();
             if (st) return fail(PL_ERR_STATE, "device carry did not settle");

@@ -141,9 +141,10 @@ class Group {
     // inverse column pass, [3] flat carry, [4] block fix-up, [5] inverse
     // column DFT + carry, [6] carry-in + forward column DFT ([5], [6] are 0
     // without the split outer pass; with it, [0], [2]-[4] come from the one
-    // entry/exit pass around the loop).  Slot 0's residue is saved and
-    // restored; other slots are squared `reps` times.
-    void kernel_times(int reps, double* ms);
+    // entry/exit pass around the loop).  The kernels cover slots
+    // [0, nslots); slot 0's residue is saved and restored, the others are
+    // squared `reps` times.
+    void kernel_times(int reps, double* ms, uint32_t nslots);
     static constexpr int kNumKernels = 8;
     // squarings per plain chunk (one graph replay, one round-off read-back)
     // and columns of the round-off arrays; a chunk enters and leaves the

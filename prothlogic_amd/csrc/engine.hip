@@ -1425,7 +1425,10 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
 {
     if (capacity == 0) throw std::invalid_argument("group capacity must be positive");
     check(hipSetDevice(device), "hipSetDevice");
-    check(hipStreamCreate(&stream_), "hipStreamCreate");
+    // non-blocking: no implicit ordering with the legacy stream, so host
+    // threads driving other groups (and capturing their graphs) never
+    // depend on this one
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     std::vector<double2> tw, twl_lo, twl_hi, om_lo, om_hi;
     std::vector<double> pw_lo, pw_hi, ipw_lo, ipw_hi;
     host_tables(p, tw, twl_lo, twl_hi, om_lo, om_hi, pw_lo, pw_hi, ipw_lo, ipw_hi);
@@ -1531,7 +1534,9 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
     check(hipMemsetAsync(mem_, 0, zero_end, stream_), "hipMemsetAsync");
     check(hipStreamSynchronize(stream_), "init sync");
     desc_host_.assign(cap_, SlotDesc{ p.n, (long long)p.k, 0, 0, 0 });
-    check(hipMemcpy(desc_, desc_host_.data(), sizeof(SlotDesc) * cap_, hipMemcpyHostToDevice), "desc upload");
+    check(hipMemcpyAsync(desc_, desc_host_.data(), sizeof(SlotDesc) * cap_, hipMemcpyHostToDevice, stream_),
+          "desc upload");
+    check(hipStreamSynchronize(stream_), "desc sync");
     err_host_.resize((size_t)kChunk * cap_);
 }
 
@@ -2026,8 +2031,9 @@ std::vector<SlotState> Group::states()
     return v;
 }
 
-void Group::kernel_times(int reps, double* ms)
+void Group::kernel_times(int reps, double* ms, uint32_t nslots)
 {
+    if (nslots == 0 || nslots > cap_) throw std::invalid_argument("kernel_times: slot count out of range");
     check(hipSetDevice(device_), "hipSetDevice");
     std::vector<int32_t> saved;
     int64_t saved_xhi = 0;
@@ -2057,24 +2063,24 @@ void Group::kernel_times(int reps, double* ms)
     // enter: the first squaring's forward transform (timed once, then the
     // forward column pass as the unfused steady state times it)
     check(hipEventRecord(ev[0], stream_), "hipEventRecord");
-    launch(kFwdCol, p, cap_);
+    launch(kFwdCol, p, nslots);
     check(hipEventRecord(ev[1], stream_), "hipEventRecord");
-    launch(kRow, p, cap_);
+    launch(kRow, p, nslots);
     size_t e = 2;
     check(hipEventRecord(ev[e], stream_), "hipEventRecord");
     const size_t loop0 = e;
     for (int r = 0; r < reps; ++r)
         for (int j = 0; j < m; ++j) {
-            launch(seq[(size_t)j], seq[(size_t)j] == kCarryTurn ? pt : p, cap_);
+            launch(seq[(size_t)j], seq[(size_t)j] == kCarryTurn ? pt : p, nslots);
             check(hipEventRecord(ev[++e], stream_), "hipEventRecord");
         }
     // leave: back to digits
     const size_t tail0 = e;
-    launch(kInvCol, p, cap_);
+    launch(kInvCol, p, nslots);
     check(hipEventRecord(ev[++e], stream_), "hipEventRecord");
-    launch(kCarry, p, cap_);
+    launch(kCarry, p, nslots);
     check(hipEventRecord(ev[++e], stream_), "hipEventRecord");
-    launch(kFixup, p, cap_);
+    launch(kFixup, p, nslots);
     check(hipEventRecord(ev[++e], stream_), "hipEventRecord");
     check(hipEventSynchronize(ev[e]), "hipEventSynchronize");
     auto span = [&](size_t i, size_t j) {
@@ -2095,15 +2101,15 @@ void Group::kernel_times(int reps, double* ms)
         hipEvent_t a, b;
         check(hipEventCreateWithFlags(&a, hipEventDisableSystemFence), "hipEventCreate");
         check(hipEventCreateWithFlags(&b, hipEventDisableSystemFence), "hipEventCreate");
-        launch(kFwdCol, p, cap_);
-        launch(kRow, p, cap_);
+        launch(kFwdCol, p, nslots);
+        launch(kRow, p, nslots);
         check(hipEventRecord(a, stream_), "hipEventRecord");
         for (int r = 0; r < reps; ++r)
-            for (int j = 0; j < m; ++j) launch(seq[(size_t)j], seq[(size_t)j] == kCarryTurn ? pt : p, cap_);
+            for (int j = 0; j < m; ++j) launch(seq[(size_t)j], seq[(size_t)j] == kCarryTurn ? pt : p, nslots);
         check(hipEventRecord(b, stream_), "hipEventRecord");
-        launch(kInvCol, p, cap_);
-        launch(kCarry, p, cap_);
-        launch(kFixup, p, cap_);
+        launch(kInvCol, p, nslots);
+        launch(kCarry, p, nslots);
+        launch(kFixup, p, nslots);
         check(hipEventSynchronize(b), "hipEventSynchronize");
         float t = 0;
         check(hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
@@ -2128,7 +2134,8 @@ void Group::kernel_times(int reps, double* ms)
 int Group::status()
 {
     int s = 0;
-    check(hipMemcpy(&s, status_, sizeof(int), hipMemcpyDeviceToHost), "status D2H");
+    check(hipMemcpyAsync(&s, status_, sizeof(int), hipMemcpyDeviceToHost, stream_), "status D2H");
+    check(hipStreamSynchronize(stream_), "status sync");
     return s;
 }
 
@@ -2154,7 +2161,7 @@ double copy_probe(int device, size_t bytes, int reps)
     hipEvent_t t0 = nullptr, t1 = nullptr;
     float ms = 0;
     try {
-        check(hipStreamCreate(&st), "hipStreamCreate");
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreate(&t0), "hipEventCreate");
         check(hipEventCreate(&t1), "hipEventCreate");
         check(hipMemsetAsync(a, 0, n * sizeof(double2), st), "hipMemsetAsync");

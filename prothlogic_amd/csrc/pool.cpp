@@ -21,6 +21,7 @@
 
 #include "bignum.h"
 #include "engine.h"
+#include "prewrap.h"
 #include "prothcore.h"
 
 namespace plpool {
@@ -56,7 +57,8 @@ struct Occupant {
 struct LenGroup {
     pleng::Plan plan;
     std::unique_ptr<pleng::Group> g;
-    std::vector<Occupant> slots;
+    std::vector<Occupant> slots;  // the group's test slots; three scratch slots follow them
+    plprep::Scratch scr;
 };
 
 }  // namespace plpool
@@ -110,17 +112,79 @@ LenGroup* group_for(pl_pool* P, uint32_t n, uint64_t k, std::string& why)
     }
     auto lg = std::make_unique<LenGroup>();
     lg->plan = pl;
-    lg->g = std::make_unique<pleng::Group>(pl, cap, P->device);
+    lg->g = std::make_unique<pleng::Group>(pl, cap + 3, P->device);
     lg->g->no_turn();  // groups step concurrently on their own streams
     lg->slots.resize(cap);
+    lg->scr = plprep::Scratch{ cap, cap + 1, cap + 2 };
     P->used_bytes += lg->g->device_bytes();
     LenGroup* raw = lg.get();
     P->groups[pl.logL] = std::move(lg);
     return raw;
 }
 
+pl_pool_result void_result(pl_pool* P, const Request& q, uint32_t L, int error, double max_err);
+void evict(pl_pool* P, LenGroup& lg, uint32_t s);
+
+pl_pool_result finished_result(pl_pool* P, const Occupant& o, uint32_t L, const plbn::Limbs& x)
+{
+    pl_pool_result r{};
+    r.tag = o.req.tag;
+    r.k = o.req.k;
+    r.n = o.req.n;
+    r.fft_len = L;
+    r.base = o.base;
+    r.res64 = plbn::low64(x);
+    r.max_err = o.max_err;
+    r.squarings = o.total;
+    const bool full = !o.req.stop_after || o.req.stop_after >= (uint64_t)o.req.n - 1;
+    r.verdict = full ? (plbn::is_n_minus_1(*o.M, x) ? 1 : 0) : -1;
+    r.gated = 0;
+    r.error = PL_OK;
+    r.t_submit = o.req.t_submit;
+    r.t_admit = o.t_admit;
+    r.t_done = P->now();
+    return r;
+}
+
+// The first squarings of an admitted test, while its exact value is small:
+// the seed a^k and then squarings until the value has wrapped modulo N, in
+// normal form on the slot (prewrap.h), then the move to the Montgomery form
+// the scheduled steps use.  Returns the squarings applied; all of them
+// (o.total) when the test ends before its value wraps, the slot then still
+// holding the value itself.
+uint64_t prewrap(LenGroup& lg, uint32_t s, Occupant& o, bool& normal)
+{
+    pleng::Group& G = *lg.g;
+    const plbn::Modulus& M = *o.M;
+    for (uint32_t t : { lg.scr.t, lg.scr.m1, lg.scr.ma }) G.set_slot(t, o.req.n, o.req.k, 0, 0);
+    plprep::load_constants(G, M, o.base, lg.scr);
+    plprep::SmallValue v;
+    normal = v.set(M, plbn::Limbs(1, o.base));
+    plprep::set_raw(G, M, s, plbn::Limbs(1, o.base));
+    const unsigned top = topbit64(o.req.k);
+    uint64_t done = 0;
+    for (int b = (int)top - 1; b >= 0 && done < o.total; --b, ++done) {
+        const bool bit = ((o.req.k >> b) & 1) != 0;
+        o.max_err = std::max(o.max_err, plprep::step(G, s, lg.scr, bit));
+        v.grow(true, bit ? o.base : 0);
+    }
+    while (!v.wrapped(M) && done < o.total) {
+        o.max_err = std::max(o.max_err, plprep::step(G, s, lg.scr, false));
+        v.grow(true, 0);
+        ++done;
+    }
+    if (done < o.total) {
+        o.max_err = std::max(o.max_err, plprep::to_mont(G, s, lg.scr));
+        normal = false;
+    }
+    return done;
+}
+
 void admit(pl_pool* P)
 {
+    // slots whose carry did not settle during their first squarings; evicted
+    // (and requeued) after the walk over the queue
+    std::vector<std::pair<LenGroup*, uint32_t>> failed;
     for (auto it = P->queue.begin(); it != P->queue.end();) {
         const Request r = *it;
         std::string why;
@@ -149,15 +213,30 @@ void admit(pl_pool* P)
         const unsigned top = topbit64(r.k);
         const uint64_t sq = r.stop_after ? std::min<uint64_t>(r.stop_after, r.n - 1) : r.n - 1;
         o.total = top + sq;
-        std::vector<int32_t> d;
-        int64_t xhi = 0;
-        plbn::to_digits(*o.M, lg->plan.L, plbn::to_mont(*o.M, plbn::Limbs(1, o.base ? o.base : 0)), d, xhi);
-        lg->g->set_slot((uint32_t)free_slot, r.n, r.k, (uint32_t)o.base, o.total);
-        lg->g->upload((uint32_t)free_slot, d, xhi);
-        lg->g->set_state((uint32_t)free_slot, 0, o.total > 1);
-        lg->g->enter((uint32_t)free_slot);
+        const uint32_t s = (uint32_t)free_slot;
+        lg->g->set_slot(s, r.n, r.k, (uint32_t)o.base, o.total);
+        bool normal = false;
+        const uint64_t done = prewrap(*lg, s, o, normal);
         it = P->queue.erase(it);
+        if (lg->g->status()) {
+            lg->g->clear_status();
+            failed.emplace_back(lg, s);
+            continue;
+        }
+        if (done >= o.total) {
+            // the whole test ran before its value wrapped: the slot holds x
+            const plbn::Limbs x = plprep::raw(*lg->g, *o.M, s);
+            if (!(o.max_err <= kErrGuard)) P->done.push_back(void_result(P, o.req, lg->plan.L, PL_ERR_ROUNDOFF, o.max_err));
+            else P->done.push_back(finished_result(P, o, lg->plan.L, normal ? x : plbn::from_mont(*o.M, x)));
+            o.used = false;
+            o.M.reset();
+            continue;
+        }
+        // the scheduled steps carry the rest (the seed's multipliers are done)
+        lg->g->set_state(s, done, done + 1 < o.total);
+        lg->g->enter(s);
     }
+    for (auto& f : failed) evict(P, *f.first, f.second);
 }
 
 pl_pool_result void_result(pl_pool* P, const Request& q, uint32_t L, int error, double max_err)
@@ -220,26 +299,8 @@ void retire(pl_pool* P)
                 o.M.reset();
                 continue;
             }
-            std::vector<int32_t> d;
-            int64_t xhi = 0;
-            lg.g->download((uint32_t)s, d, xhi);
-            const plbn::Limbs x = plbn::from_mont(*o.M, plbn::from_digits(*o.M, lg.plan.L, d, xhi));
-            pl_pool_result r{};
-            r.tag = o.req.tag;
-            r.k = o.req.k;
-            r.n = o.req.n;
-            r.fft_len = lg.plan.L;
-            r.base = o.base;
-            r.res64 = plbn::low64(x);
-            r.max_err = o.max_err;
-            r.squarings = o.total;
-            const bool full = !o.req.stop_after || o.req.stop_after >= (uint64_t)o.req.n - 1;
-            r.verdict = full ? (plbn::is_n_minus_1(*o.M, x) ? 1 : 0) : -1;
-            r.gated = 0;
-            r.error = PL_OK;
-            r.t_submit = o.req.t_submit;
-            r.t_admit = o.t_admit;
-            r.t_done = P->now();
+            const plbn::Limbs x = plbn::from_mont(*o.M, plprep::raw(*lg.g, *o.M, (uint32_t)s));
+            const pl_pool_result r = finished_result(P, o, lg.plan.L, x);
             P->done.push_back(r);
             o.used = false;
             o.M.reset();
@@ -329,7 +390,8 @@ int pl_pool_run(pl_pool* P, uint32_t steps, pl_pool_result* out, size_t cap, siz
             const uint32_t g = std::min<uint32_t>(pleng::Group::kStepChunk, steps - done);
             for (size_t i = 0; i < live.size(); ++i) live[i]->g->step_begin(g, span[i]);
             for (LenGroup* lg : live) {
-                std::vector<double> se(lg->slots.size(), 0.0);
+                // step_end reports every slot of the group, scratch slots included
+                std::vector<double> se(lg->g->capacity(), 0.0);
                 lg->g->step_end(se.data());
                 for (size_t s = 0; s < lg->slots.size(); ++s)
                     if (lg->slots[s].used) lg->slots[s].max_err = std::max(lg->slots[s].max_err, se[s]);

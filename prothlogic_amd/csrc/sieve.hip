@@ -363,7 +363,7 @@ int pl_sieve_range(uint64_t k_start, uint64_t k_end, uint32_t n, uint64_t pmax, 
     if (rc) return rc;
     hipStream_t st = nullptr;
     try {
-        hcheck(hipStreamCreate(&st), "hipStreamCreate");
+        hcheck(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
         const uint64_t nbits = k_end - k_start + 1;
         const uint64_t nwords = (nbits + 31) / 32;
         DevBuf bm(nwords * 4);
@@ -417,7 +417,7 @@ int pl_sieve_list(const uint64_t* ks, size_t count, uint32_t n, uint64_t pstart,
     if (rc) return rc;
     hipStream_t st = nullptr;
     try {
-        hcheck(hipStreamCreate(&st), "hipStreamCreate");
+        hcheck(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
         DevBuf dk(count * 8), dkeep(count);
         hcheck(hipMemcpyAsync(dk.p, ks, count * 8, hipMemcpyHostToDevice, st), "H2D ks");
         hcheck(hipMemcpyAsync(dkeep.p, keep, count, hipMemcpyHostToDevice, st), "H2D keep");
