@@ -118,6 +118,22 @@ int pl_is_n_minus_1(pl_ctx* ctx, int* is_nm1);
 int pl_fft_square_karatsuba(pl_ctx* ctx, uint64_t* v, size_t n_limbs, unsigned n_bits, uint64_t k,
                             double* last_max_diff);
 
+/* No counterpart in the reference: pl_fft_square_karatsuba for callers
+ * that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  v[0..n_limbs) (host LE
+ * u64 limbs) <- v^2 mod (k*2^n_bits+1), *last_max_diff (nullable) <- the
+ * squaring's max round-off, with no context: every device buffer the call
+ * uses lies in one static array per GPU (6 MiB), so the call makes no
+ * device allocation (hipMalloc, hipFree, stream or graph creation; host
+ * conversion buffers are ordinary heap memory) and runs on the legacy
+ * stream.  n_bits must need a transform length of at most 2^18 (about
+ * 4.5 million bits): longer ones fail with PL_ERR_ARG.  Calls on one device
+ * are serialised within a process and share its array, which keeps the
+ * last call's contents (each call clears what it reads).  Errors are
+ * described by pl_last_error(). */
+int pl_fft_square_karatsuba_noalloc(int device, uint64_t* v, size_t n_limbs, unsigned n_bits, uint64_t k,
+                                    double* last_max_diff);
+
 /* Diagnostics (no counterpart in the reference): average duration in ms of
  * each kernel of one squaring, timed with HIP events on the context's own
  * stream between consecutive kernels of `reps` real squarings run in their

@@ -90,6 +90,7 @@ def lib() -> ctypes.CDLL:
         "pl_res64": ([vp, _u64p], i32),
         "pl_is_n_minus_1": ([vp, ctypes.POINTER(i32)], i32),
         "pl_fft_square_karatsuba": ([vp, _u64p, sz, ctypes.c_uint, u64, _dp], i32),
+        "pl_fft_square_karatsuba_noalloc": ([i32, _u64p, sz, ctypes.c_uint, u64, _dp], i32),
         "pl_kernel_times": ([vp, i32, _dp, _u64p], i32),
         "pl_copy_probe": ([i32, u64, i32, _dp], i32),
         "pl_autotune_fft_len": ([u32, u64, i32, dbl, ctypes.POINTER(u32), _dp, _dp], i32),
@@ -246,6 +247,15 @@ class Context:
         e = ctypes.c_double(0.0)
         _check(lib().pl_fft_square_karatsuba(self.h, arr, n_limbs, self.n, self.k, ctypes.byref(e)))
         return list(arr), e.value
+
+
+def fft_square_karatsuba_noalloc(limbs, n_limbs: int, n: int, k: int, device: int = 0):
+    """v <- v^2 mod k*2^n+1 with no device allocation (the static scratch
+    array; pl_fft_square_karatsuba_noalloc).  Returns (limbs, round-off)."""
+    arr = (ctypes.c_uint64 * n_limbs)(*limbs)
+    e = ctypes.c_double(0.0)
+    _check(lib().pl_fft_square_karatsuba_noalloc(device, arr, n_limbs, n, k, ctypes.byref(e)))
+    return list(arr), e.value
 
 
 def proth_test(k: int, n: int, fft_len: int = 0, device: int = 0, checkpoint_path: str | None = None,

@@ -19,14 +19,20 @@
 #include "prewrap.h"
 
 namespace {
-
 thread_local std::string g_err = "no error";
+}  // namespace
 
+namespace plerr {
 int fail(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
 }
+}  // namespace plerr
+
+namespace {
+
+using plerr::fail;
 
 template <class F>
 int guarded(F&& f)

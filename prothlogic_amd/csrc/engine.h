@@ -59,7 +59,13 @@ struct SlotState {
 class Group {
   public:
     Group(const Plan& p, uint32_t capacity, int device);
+    // A group in memory the caller provides (at least layout_bytes(p,
+    // capacity) bytes of device memory on `device`) and on the caller's
+    // stream: nothing is allocated on the device and nothing is freed at
+    // destruction.  Such a group takes the split outer pass only.
+    Group(const Plan& p, uint32_t capacity, int device, void* mem, size_t mem_bytes, hipStream_t stream);
     ~Group();
+    static size_t layout_bytes(const Plan& p, uint32_t capacity);
     Group(const Group&) = delete;
     Group& operator=(const Group&) = delete;
 
@@ -154,6 +160,7 @@ class Group {
     static constexpr int kStepChunk = 64;
 
   private:
+    void init(void* ext_mem, size_t ext_bytes, hipStream_t ext_stream);
     Params params(uint32_t slot0, bool check_active, uint32_t err_col) const;
     void launch(int kind, const Params& p, uint32_t slots);
     void enqueue_plain_chunk(int g, bool use_turn);
@@ -204,6 +211,7 @@ class Group {
     bool turn_ = true;     // split outer pass as one launch (k_carry_turn); PL_CARRY_TURN=0: two
     bool give_up_ = false; // PL_HANDOFF_GIVE_UP=1: every carry hand-off wait gives up (tests)
     uint32_t fallbacks_ = 0;
+    bool owns_ = true;     // the memory and the stream are the group's own
     std::vector<unsigned long long> err_host_;
 };
 

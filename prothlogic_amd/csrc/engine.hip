@@ -1421,58 +1421,111 @@ size_t Group::slot_bytes(const Plan& p)
            (size_t)(p.R + p.C) * 32 + 64 + kChunk * 8;
 }
 
+// Byte offsets of a group's buffers in its one device allocation.
+struct GroupLayout {
+    size_t z, car, runc, runt, epoch, rdig, xhi, err, errp, st, dig, desc, state, zero_end;
+    size_t tw, tlo, thi, olo, ohi, plo, phi, ilo, ihi, tcq, wrf, wri, wcf, wci, bytes;
+    uint32_t carry_tpb, carry_blocks, eb;
+};
+
+static GroupLayout group_layout(const Plan& p, uint32_t cap)
+{
+    GroupLayout g{};
+    g.carry_tpb = carry_tpb_for(p);
+    g.carry_blocks = p.L / (p.run * g.carry_tpb);
+    g.eb = std::max<uint32_t>(g.carry_blocks, p.nt ? p.C / p.nt : 1);
+    const size_t runs = p.nt ? (size_t)p.L / p.nt : 0;
+    const size_t S = (size_t)1 << p.s, H = p.L / S;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t off = 0;
+    auto put = [&](size_t& at, size_t bytes) {
+        at = off;
+        off += al(bytes);
+    };
+    put(g.z, sizeof(double2) * p.L * cap);
+    put(g.car, sizeof(long long) * 3 * g.carry_blocks * cap);
+    put(g.runc, sizeof(long long) * 3 * runs * cap);
+    put(g.runt, p.nt == 8 ? sizeof(long long) * 3 * runs * cap : 0);
+    put(g.epoch, sizeof(unsigned) * 2 * cap);
+    put(g.rdig, p.nt ? sizeof(int) * (size_t)p.L * cap : 0);
+    put(g.xhi, sizeof(long long) * cap);
+    put(g.err, sizeof(unsigned long long) * Group::kChunk * cap);
+    put(g.errp, sizeof(unsigned long long) * Group::kChunk * g.eb * cap);
+    put(g.st, sizeof(int));
+    put(g.dig, sizeof(int32_t) * p.L);
+    put(g.desc, sizeof(SlotDesc) * cap);
+    put(g.state, sizeof(SlotState) * cap);
+    g.zero_end = off;
+    put(g.tw, sizeof(double2) * 4096);
+    put(g.tlo, sizeof(double2) * S);
+    put(g.thi, sizeof(double2) * H);
+    put(g.olo, sizeof(double2) * S);
+    put(g.ohi, sizeof(double2) * H);
+    put(g.plo, sizeof(double) * S);
+    put(g.phi, sizeof(double) * H);
+    put(g.ilo, sizeof(double) * S);
+    put(g.ihi, sizeof(double) * H);
+    put(g.tcq, sizeof(double2) * (size_t)p.C * 16);
+    put(g.wrf, sizeof(double2) * p.R * cap);
+    put(g.wri, sizeof(double2) * p.R * cap);
+    put(g.wcf, sizeof(double2) * p.C * cap);
+    put(g.wci, sizeof(double2) * p.C * cap);
+    g.bytes = off;
+    return g;
+}
+
+size_t Group::layout_bytes(const Plan& p, uint32_t capacity) { return group_layout(p, capacity).bytes; }
+
 Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capacity), device_(device)
 {
-    if (capacity == 0) throw std::invalid_argument("group capacity must be positive");
-    check(hipSetDevice(device), "hipSetDevice");
-    // non-blocking: no implicit ordering with the legacy stream, so host
-    // threads driving other groups (and capturing their graphs) never
-    // depend on this one
-    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    init(nullptr, 0, nullptr);
+}
+
+Group::Group(const Plan& p, uint32_t capacity, int device, void* mem, size_t mem_bytes, hipStream_t stream)
+    : plan_(p), cap_(capacity), device_(device)
+{
+    if (!mem) throw std::invalid_argument("external group memory is NULL");
+    init(mem, mem_bytes, stream);
+}
+
+void Group::init(void* ext_mem, size_t ext_bytes, hipStream_t ext_stream)
+{
+    const Plan& p = plan_;
+    if (cap_ == 0) throw std::invalid_argument("group capacity must be positive");
+    check(hipSetDevice(device_), "hipSetDevice");
+    const GroupLayout lay = group_layout(p, cap_);
+    if (ext_mem) {
+        if (ext_bytes < lay.bytes) throw std::invalid_argument("external group memory too small for this length");
+        owns_ = false;
+        stream_ = ext_stream;
+        mem_ = ext_mem;
+        turn_ = false;  // the one-launch pass's fallback would allocate its backup
+    } else {
+        // non-blocking: no implicit ordering with the legacy stream, so host
+        // threads driving other groups (and capturing their graphs) never
+        // depend on this one
+        check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    }
     std::vector<double2> tw, twl_lo, twl_hi, om_lo, om_hi;
     std::vector<double> pw_lo, pw_hi, ipw_lo, ipw_hi;
     host_tables(p, tw, twl_lo, twl_hi, om_lo, om_hi, pw_lo, pw_hi, ipw_lo, ipw_hi);
-    carry_tpb_ = carry_tpb_for(p);
-    carry_blocks_ = p.L / (p.run * carry_tpb_);
-    if (const char* e = std::getenv("PL_CARRY_TURN")) turn_ = e[0] != '0';
+    carry_tpb_ = lay.carry_tpb;
+    carry_blocks_ = lay.carry_blocks;
+    if (owns_) {
+        if (const char* e = std::getenv("PL_CARRY_TURN")) turn_ = e[0] != '0';
+    }
     if (const char* e = std::getenv("PL_HANDOFF_GIVE_UP")) give_up_ = e[0] == '1';
-    const size_t runs = p.nt ? (size_t)p.L / p.nt : 0;
-
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    size_t off = 0;
-    const size_t o_z = off; off += al(sizeof(double2) * p.L * cap_);
-    const size_t o_car = off; off += al(sizeof(long long) * 3 * carry_blocks_ * cap_);
-    const size_t o_runc = off; off += al(sizeof(long long) * 3 * runs * cap_);
-    const size_t o_runt = off; off += al(p.nt == 8 ? sizeof(long long) * 3 * runs * cap_ : 0);
-    const size_t o_epoch = off; off += al(sizeof(unsigned) * 2 * cap_);
-    const size_t o_rdig = off; off += al(p.nt ? sizeof(int) * (size_t)p.L * cap_ : 0);
-    const size_t o_xhi = off; off += al(sizeof(long long) * cap_);
-    eb_ = std::max<uint32_t>(carry_blocks_, p.nt ? p.C / p.nt : 1);
-    const size_t o_err = off; off += al(sizeof(unsigned long long) * kChunk * cap_);
-    const size_t o_errp = off; off += al(sizeof(unsigned long long) * kChunk * eb_ * cap_);
-    const size_t o_st = off; off += al(sizeof(int));
-    const size_t o_dig = off; off += al(sizeof(int32_t) * p.L);
-    const size_t o_desc = off; off += al(sizeof(SlotDesc) * cap_);
-    const size_t o_state = off; off += al(sizeof(SlotState) * cap_);
-    const size_t zero_end = off;
-    const size_t o_tw = off; off += al(sizeof(double2) * tw.size());
-    const size_t o_tlo = off; off += al(sizeof(double2) * twl_lo.size());
-    const size_t o_thi = off; off += al(sizeof(double2) * twl_hi.size());
-    const size_t o_olo = off; off += al(sizeof(double2) * om_lo.size());
-    const size_t o_ohi = off; off += al(sizeof(double2) * om_hi.size());
-    const size_t o_plo = off; off += al(sizeof(double) * pw_lo.size());
-    const size_t o_phi = off; off += al(sizeof(double) * pw_hi.size());
-    const size_t o_ilo = off; off += al(sizeof(double) * ipw_lo.size());
-    const size_t o_ihi = off; off += al(sizeof(double) * ipw_hi.size());
+    eb_ = lay.eb;
+    const size_t o_z = lay.z, o_car = lay.car, o_runc = lay.runc, o_runt = lay.runt, o_epoch = lay.epoch;
+    const size_t o_rdig = lay.rdig, o_xhi = lay.xhi, o_err = lay.err, o_errp = lay.errp, o_st = lay.st;
+    const size_t o_dig = lay.dig, o_desc = lay.desc, o_state = lay.state, zero_end = lay.zero_end;
+    const size_t o_tw = lay.tw, o_tlo = lay.tlo, o_thi = lay.thi, o_olo = lay.olo, o_ohi = lay.ohi;
+    const size_t o_plo = lay.plo, o_phi = lay.phi, o_ilo = lay.ilo, o_ihi = lay.ihi, o_tcq = lay.tcq;
+    const size_t o_wrf = lay.wrf, o_wri = lay.wri, o_wcf = lay.wcf, o_wci = lay.wci;
     std::vector<double2> tcq;
     host_tcq(p, tcq);
-    const size_t o_tcq = off; off += al(sizeof(double2) * tcq.size());
-    const size_t o_wrf = off; off += al(sizeof(double2) * p.R * cap_);
-    const size_t o_wri = off; off += al(sizeof(double2) * p.R * cap_);
-    const size_t o_wcf = off; off += al(sizeof(double2) * p.C * cap_);
-    const size_t o_wci = off; off += al(sizeof(double2) * p.C * cap_);
-    bytes_ = off;
-    check(hipMalloc(&mem_, bytes_), "hipMalloc");
+    bytes_ = lay.bytes;
+    if (owns_) check(hipMalloc(&mem_, bytes_), "hipMalloc");
     char* b = static_cast<char*>(mem_);
     z_ = reinterpret_cast<double2*>(b + o_z);
     carries_ = reinterpret_cast<long long*>(b + o_car);
@@ -1543,7 +1596,7 @@ Group::Group(const Plan& p, uint32_t capacity, int device) : plan_(p), cap_(capa
 Group::~Group()
 {
     // nothing launched on the stream may still use the graphs or the memory
-    if (stream_) (void)hipStreamSynchronize(stream_);
+    (void)hipStreamSynchronize(stream_);
     for (auto& kv : plain_graphs_) {
         (void)hipGraphExecDestroy(kv.second.first);
         (void)hipGraphDestroy(kv.second.second);
@@ -1552,9 +1605,11 @@ Group::~Group()
         (void)hipGraphExecDestroy(kv.second.first);
         (void)hipGraphDestroy(kv.second.second);
     }
-    if (mem_) (void)hipFree(mem_);
-    if (zbak_) (void)hipFree(zbak_);
-    if (stream_) (void)hipStreamDestroy(stream_);
+    if (owns_) {
+        if (mem_) (void)hipFree(mem_);
+        if (zbak_) (void)hipFree(zbak_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
     delete tabs_;
 }
 
