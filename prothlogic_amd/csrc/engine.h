@@ -28,10 +28,10 @@ struct Plan {
     uint32_t run = 16;    // digits per thread in the flat carry kernel (8 or 16)
 };
 
-// Smallest power of two L in [2^10, 2^22] with n/L <= max_bits_per_digit(L)
-// and n/L >= 8.  force_L, when nonzero, is checked and used instead.
+// Smallest power of two L in [2^10, 2^22] with n/L <= max_bits_per_digit(L, k)
+// and n/L >= 7.  force_L, when nonzero, is checked and used instead.
 bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::string& why);
-double max_bits_per_digit(uint32_t logL);
+double max_bits_per_digit(uint32_t logL, uint64_t k);
 // Fills nt and run for slots whose bits per digit are at least bmin, whose
 // k is below kmax and whose per-squaring multiplier is at most mult_max.
 // Returns false when no carry layout can absorb those carries.
@@ -161,16 +161,19 @@ class Group {
 
   private:
     void init(void* ext_mem, size_t ext_bytes, hipStream_t ext_stream);
-    Params params(uint32_t slot0, bool check_active, uint32_t err_col) const;
+    Params params(uint32_t slot0, bool check_active) const;
     void launch(int kind, const Params& p, uint32_t slots);
     void enqueue_plain_chunk(int g, bool use_turn);
+    void enqueue_exit(const Params& p, uint32_t slots);
     void enqueue_step_chunk(int g);
     void build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g, bool use_turn);
     void run_plain_chunk(int g, bool use_turn);
-    void save_start(uint32_t slot0, uint32_t slots);
+    void ensure_backup();
     void restore_start(uint32_t slot0, uint32_t slots);
+    void sync_status(const char* what);
     double read_err(int g, double* trace, double* slot_err, uint32_t trace_slot = 0, uint32_t nslots = 0);
-    void enqueue_err_clear(uint32_t slots);
+    void enqueue_err_clear(uint32_t slots, unsigned col0);
+    void enqueue_set_col(uint32_t slot, unsigned col);
     void enqueue_err_reduce(uint32_t slot0, uint32_t slots);
 
     Plan plan_;
@@ -212,6 +215,12 @@ class Group {
     bool give_up_ = false; // PL_HANDOFF_GIVE_UP=1: every carry hand-off wait gives up (tests)
     uint32_t fallbacks_ = 0;
     bool owns_ = true;     // the memory and the stream are the group's own
+    // the status word as read at the last wait on the stream, and whether
+    // kernels were launched since
+    int seen_status_ = 0;
+    bool launched_ = true;
+    int status_word_ = 0;
+    int* status_host_ = &status_word_;
     std::vector<unsigned long long> err_host_;
 };
 

@@ -87,11 +87,13 @@ struct Params {
     long long* carries;     // flat carry kernel: 3 per block (own carry-outs of its last run)
     long long* runc;        // split outer pass: 3 carry-outs per run (block-major, then row);
                             // the one-launch pass stores them as tagged words
-    unsigned* epoch;        // per slot: [0] the next hand-off epoch, [1] the last one used
+    unsigned* epoch;        // per slot: [0] the next hand-off epoch, [1] the last one used,
+                            // [2] the round-off column of the squaring the next outer pass
+                            // carries (the row pass advances it), [3] unused
     int* dig;               // split outer pass: NT int32 digits per run (same order)
     long long* xhi;
     const long long* xhi_mul;  // product pass: the other factor's X_hi (nullptr: squaring)
-    unsigned long long* err;  // per-block round-off partials: block b of this squaring at err[b*kChunk]
+    unsigned long long* err;  // per-block round-off partials: block b, column c at err[b*kChunk + c]
     int* status;
     const SlotDesc* desc;
     SlotState* state;
@@ -134,7 +136,7 @@ __device__ __forceinline__ Params bind(const Params& base, unsigned y)
     p.z += (size_t)slot * base.L;
     p.dig += (size_t)slot * base.L;
     p.runc += (size_t)slot * base.runc_stride;
-    p.epoch += 2 * (size_t)slot;
+    p.epoch += 4 * (size_t)slot;
     p.carries += (size_t)slot * base.carr_stride;
     p.xhi += slot;
     p.err += (size_t)slot * base.err_stride;  // err_stride = blocks * kChunk
@@ -346,8 +348,43 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
+// Four-step twiddles W_L^(c*idx) of the points one thread moves in the
+// first (or last) stage of a column DFT, where idx = j + r*R/P:
+// W_L^(c*j) (two-level table) times W_L^(c*r*R/P) = q^r with
+// q = tcq[c*16 + 16/P], powered by multiplication (one table load per
+// column instead of P-1; 0.5 us per squaring at n = 2^21).
+template <int R, int NT, int TPB, int P>
+struct ColTw {
+    using S = StageShape<R, NT, TPB, P>;
+    double2 w[S::PER][P];
+    // SCALED: every factor also carries f, a factor that is constant down
+    // column c (k_carry_turn folds the column weight in here: the column DFT
+    // is linear, so scaling its input or output by f scales the whole column)
+    template <bool SCALED = false>
+    __device__ __forceinline__ void load(const Params& p, unsigned c, double2 f = make_double2(1.0, 0.0))
+    {
+#pragma unroll
+        for (int u = 0; u < S::PER; ++u) {
+            if (!S::live(u)) continue;
+            double2 a = twL(p, c * (unsigned)S::j(u));
+            if constexpr (SCALED) a = cmul(a, f);
+            const double2 q = p.tcq[(size_t)c * 16 + 16 / P];
+            w[u][0] = a;
+            double2 qr = q;
+#pragma unroll
+            for (int r = 1; r < P; ++r) {
+                w[u][r] = cmul(a, qr);
+                if (r + 1 < P) qr = cmul(qr, q);
+            }
+        }
+    }
+};
+
 // ------------------------------------------------------------- FFT kernels
 
+// Column passes alone (entering the steady state from digits, and the
+// unfused outer pass).  The four-step factors of the points a thread moves
+// come from ColTw: one table load per thread instead of two per point.
 template <int R, int NT, int TPB>
 __global__ __launch_bounds__(TPB) void k_fwd_col(Params pb)
 {
@@ -357,13 +394,12 @@ __global__ __launch_bounds__(TPB) void k_fwd_col(Params pb)
     StageTw<R, NT, TPB> t;
     load_stage_tw(t, p.tw);
     const unsigned c0 = blockIdx.x * NT;
+    ColTw<R, NT, TPB, Radix<R>::last()> ct;
+    ct.load(p, c0 + threadIdx.x % NT);
     const double2* __restrict__ z = p.z;
     dft<R, NT, TPB, false>(
         lds, t, [&](int tr, int idx, int, int) { return z[(size_t)idx * p.C + c0 + tr]; },
-        [&](int tr, int idx, double2 v, int, int) {
-            const unsigned c = c0 + tr;
-            p.z[(size_t)idx * p.C + c] = cmul(v, twL(p, c * (unsigned)idx));
-        });
+        [&](int tr, int idx, double2 v, int u, int r) { p.z[(size_t)idx * p.C + c0 + tr] = cmul(v, ct.w[u][r]); });
 }
 
 template <int R, int NT, int TPB>
@@ -375,13 +411,11 @@ __global__ __launch_bounds__(TPB) void k_inv_col(Params pb)
     StageTw<R, NT, TPB> t;
     load_stage_tw(t, p.tw);
     const unsigned c0 = blockIdx.x * NT;
+    ColTw<R, NT, TPB, Radix<R>::first()> ct;
+    ct.load(p, c0 + threadIdx.x % NT);
     const double2* __restrict__ z = p.z;
     dft<R, NT, TPB, true>(
-        lds, t,
-        [&](int tr, int idx, int, int) {
-            const unsigned c = c0 + tr;
-            return cmulc(z[(size_t)idx * p.C + c], twL(p, c * (unsigned)idx));
-        },
+        lds, t, [&](int tr, int idx, int u, int r) { return cmulc(z[(size_t)idx * p.C + c0 + tr], ct.w[u][r]); },
         [&](int tr, int idx, double2 v, int, int) { p.z[(size_t)idx * p.C + c0 + tr] = v; });
 }
 
@@ -409,11 +443,15 @@ __global__ __launch_bounds__(TPB) void k_row(Params pb)
     // the row is stored (a load in the writing lane's branch alone would be
     // waited for there, in front of block 0's row loads).
     const unsigned last = __hip_atomic_load(p.epoch + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned col = __hip_atomic_load(p.epoch + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     dft_square_pass_pp<C, NT, TPB, PL_ROW_RADIX>(
         buf, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
         [&](double2 X) { return cmul(X, make_double2(X.x, X.y + x2)); },
         [&](int tr, int idx, double2 v, int, int) { p.z[base + (size_t)tr * C + idx] = v; });
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.epoch[0] = last + 1u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        p.epoch[0] = last + 1u;
+        p.epoch[2] = col + 1u;  // the next outer pass carries the next squaring
+    }
 }
 
 // Row pass of a product: this slot's row times the same row of another
@@ -450,38 +488,6 @@ __global__ __launch_bounds__(TPB) void k_row_mul(Params pb, const double2* __res
         lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; },
         [&](int tr, int idx, double2 v, int, int) { p.z[base + (size_t)tr * C + idx] = v; });
 }
-
-// Four-step twiddles W_L^(c*idx) of the points one thread moves in the
-// first (or last) stage of a column DFT, where idx = j + r*R/P:
-// W_L^(c*j) (two-level table) times W_L^(c*r*R/P) = q^r with
-// q = tcq[c*16 + 16/P], powered by multiplication (one table load per
-// column instead of P-1; 0.5 us per squaring at n = 2^21).
-template <int R, int NT, int TPB, int P>
-struct ColTw {
-    using S = StageShape<R, NT, TPB, P>;
-    double2 w[S::PER][P];
-    // SCALED: every factor also carries f, a factor that is constant down
-    // column c (k_carry_turn folds the column weight in here: the column DFT
-    // is linear, so scaling its input or output by f scales the whole column)
-    template <bool SCALED = false>
-    __device__ __forceinline__ void load(const Params& p, unsigned c, double2 f = make_double2(1.0, 0.0))
-    {
-#pragma unroll
-        for (int u = 0; u < S::PER; ++u) {
-            if (!S::live(u)) continue;
-            double2 a = twL(p, c * (unsigned)S::j(u));
-            if constexpr (SCALED) a = cmul(a, f);
-            const double2 q = p.tcq[(size_t)c * 16 + 16 / P];
-            w[u][0] = a;
-            double2 qr = q;
-#pragma unroll
-            for (int r = 1; r < P; ++r) {
-                w[u][r] = cmul(a, qr);
-                if (r + 1 < P) qr = cmul(qr, q);
-            }
-        }
-    }
-};
 
 // ------------------------------------------------------------- flat carry
 
@@ -556,7 +562,8 @@ __global__ __launch_bounds__(TPB) void k_carry(Params pb)
         double e = s_err[0];
 #pragma unroll
         for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
-        p.err[(size_t)blockIdx.x * Group::kChunk] = (unsigned long long)__double_as_longlong(e);
+        p.err[(size_t)blockIdx.x * Group::kChunk + (p.epoch[2] & (Group::kChunk - 1))] =
+            (unsigned long long)__double_as_longlong(e);
     }
 }
 
@@ -683,7 +690,8 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
         double e = s_err[0];
 #pragma unroll
         for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
-        p.err[(size_t)blockIdx.x * Group::kChunk] = (unsigned long long)__double_as_longlong(e);
+        p.err[(size_t)blockIdx.x * Group::kChunk + (p.epoch[2] & (Group::kChunk - 1))] =
+            (unsigned long long)__double_as_longlong(e);
     }
 }
 
@@ -759,6 +767,65 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
         [&](int tr, int idx, double2 v, int u, int r) {
             p.z[(size_t)idx * p.C + c0 + tr] = cmul(v, ct.w[u][r]);
         });
+}
+
+// Leaving the steady state after k_inv_carry: each run takes its
+// predecessor's carries into its first digits and the weighted digits are
+// stored in their natural places (the digit state k_get_digits and
+// k_fwd_col read), with no transform.  Block 0 replaces X_hi with the top
+// run's third carry, as k_adjust_fwd does.
+template <int R, int NT, int TPB>
+__global__ __launch_bounds__(TPB) void k_adjust_out(Params pb)
+{
+    const Params p = bind(pb, blockIdx.y);
+    if (!p.active) return;
+    constexpr int RPT = (R + TPB - 1) / TPB;
+    const unsigned nb = p.C / NT;
+    const unsigned b = blockIdx.x;
+    const unsigned c0 = b * NT;
+    const long long k = p.k;
+    const long long* top = p.runc + 3 * ((size_t)(nb - 1) * R + (R - 1));
+    double2 wc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) wc[j] = p.wcf[c0 + j];
+    const unsigned long long Lm = p.L - 1;
+    unsigned G[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) G[j] = (unsigned)((0ull - p.n * (unsigned long long)(c0 + j)) & Lm);
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+        const unsigned r = threadIdx.x + u * TPB;
+        if (r >= (unsigned)R) break;
+        const unsigned m0 = r * p.C + c0;
+        const double2 wr = p.wrf[r];
+        __int128 c2in, extra;
+        if (b == 0 && r == 0) {
+            top_rule(p, top, c2in, extra);
+        } else {
+            const long long* rc = p.runc + 3 * (b == 0 ? (size_t)(nb - 1) * R + (r - 1) : (size_t)(b - 1) * R + r);
+            c2in = (__int128)rc[1] - (__int128)k * rc[0];
+            extra = rc[2];
+        }
+        long long nu[NT];
+        ripple<NT>(p, m0, c2in, extra, nu);
+        const int4* src = reinterpret_cast<const int4*>(p.dig + ((size_t)b * R + r) * NT);
+        const unsigned F = (unsigned)((0ull - p.n * (unsigned long long)r * p.C) & Lm);
+        double2* out = p.z + m0;
+#pragma unroll
+        for (int j4 = 0; j4 < NT / 4; ++j4) {
+            const int4 q = src[j4];
+            const int dg[4] = { q.x, q.y, q.z, q.w };
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = 4 * j4 + i;
+                double2 w = cmul(wr, wc[j]);
+                if (F + G[j] >= p.L) w = make_double2(0.5 * w.x, 0.5 * w.y);
+                const double xd = (double)(dg[i] + (int)nu[j]);
+                out[j] = make_double2(xd * w.x, xd * w.y);
+            }
+        }
+    }
+    if (b == 0 && threadIdx.x == 0) *p.xhi = top[2];
 }
 
 // The outer pass in one launch: inverse column DFT, unweight, round, own
@@ -973,7 +1040,8 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         double e = s_err[0];
 #pragma unroll
         for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
-        p.err[(size_t)blockIdx.x * Group::kChunk] = (unsigned long long)__double_as_longlong(e);
+        p.err[(size_t)blockIdx.x * Group::kChunk + (p.epoch[2] & (Group::kChunk - 1))] =
+            (unsigned long long)__double_as_longlong(e);
     }
     auto ld = [&](int tr, int idx, int, int) { return io[Lay::at(tr, idx)]; };
     auto st = [&](int tr, int idx, double2 v, int u, int r) { p.z[(size_t)idx * p.C + c0 + tr] = cmul(v, ct.w[u][r]); };
@@ -998,6 +1066,25 @@ __global__ void k_get_digits(Params pb, int* __restrict__ d)
     if (m >= p.L) return;
     const double2 v = cmul(p.z[m], inv_weight(p, m));
     d[m] = (int)llrint(v.x * (double)p.L);
+}
+
+// Start of a one-launch chunk: slots' residues (count double2) and top
+// scalars (nx) copied to the backup the chunk is recomputed from if a carry
+// hand-off gives up.
+__global__ __launch_bounds__(256) void k_backup(const double2* __restrict__ z, double2* __restrict__ zb, size_t count,
+                                                const long long* __restrict__ x, long long* __restrict__ xb, unsigned nx)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t i = i0; i < count; i += stride) zb[i] = z[i];
+    if (i0 < nx) xb[i0] = x[i0];
+}
+
+// Round-off column of the next carried squaring, for slots [slot0, slot0 + n).
+__global__ void k_set_col(unsigned* epoch, unsigned slot0, unsigned n, unsigned col)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) epoch[4 * (size_t)(slot0 + i) + 2] = col;
 }
 
 __global__ void k_fill_u64(unsigned long long* a, int count)
@@ -1067,12 +1154,15 @@ constexpr int tpb_for(int points, int radix = PL_MAX_RADIX)
     return points / radix < 64 ? 64 : (points / radix > 512 ? 512 : points / radix);
 }
 
-double max_bits_per_digit(uint32_t logL)
+// The residue's top scalar X_hi (|X_hi| <~ k/2) enters the spectrum as
+// 2i*X_hi*Z, which scales the transform's relative round-off of Z by k:
+// about k * 2^(b-1) * log2(L) * 2^-53 in every coefficient.  Keeping that
+// under ~0.05 bounds b by 44.5 - log2(k); it binds only for k above 2^26.
+double max_bits_per_digit(uint32_t logL, uint64_t k)
 {
-    if (logL <= 14) return 18.5;
-    if (logL <= 17) return 18.0;
-    if (logL <= 20) return 17.5;
-    return 17.0;
+    const double table = logL <= 14 ? 18.5 : logL <= 17 ? 18.0 : logL <= 20 ? 17.5 : 17.0;
+    const double by_k = 44.5 - std::log2((double)(k ? k : 1));
+    return std::min(table, by_k);
 }
 
 static unsigned bitlen64(unsigned long long x)
@@ -1119,21 +1209,21 @@ bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::strin
             why = "fft_len must be in [2^10, 2^22]";
             return false;
         }
-        if ((double)n / (double)force_L > max_bits_per_digit(logL) + 1.0) {
+        if ((double)n / (double)force_L > max_bits_per_digit(logL, k) + 1.0) {
             why = "fft_len too short for n (round-off)";
             return false;
         }
     } else {
         for (logL = 10; logL <= 22; ++logL)
-            if ((double)n / (double)(1u << logL) <= max_bits_per_digit(logL)) break;
+            if ((double)n / (double)(1u << logL) <= max_bits_per_digit(logL, k)) break;
         if (logL > 22) {
             why = "n too large (needs fft_len > 2^22)";
             return false;
         }
     }
     const uint32_t L = 1u << logL;
-    if (n < 8u * L) {
-        why = "n too small: needs at least 8 bits per digit (n >= 8*fft_len)";
+    if (n < 7u * L) {
+        why = "n too small: needs at least 7 bits per digit (n >= 7*fft_len)";
         return false;
     }
     out.n = n;
@@ -1200,7 +1290,8 @@ enum Kind {
     kInvCarry = 5,
     kAdjFwd = 6,
     kCarryTurn = 7,
-    kAdvance = 8
+    kAdvance = 8,
+    kAdjOut = 9
 };
 
 template <int R>
@@ -1218,6 +1309,7 @@ static void launch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, h
             constexpr int TPB = SplitCfg<R, 8>::TPB;
             dim3 grid(p.C / 8, slots);
             if (kind == kInvCarry) hipLaunchKernelGGL((k_inv_carry<R, 8, TPB>), grid, dim3(TPB), 0, st, p);
+            else if (kind == kAdjOut) hipLaunchKernelGGL((k_adjust_out<R, 8, TPB>), grid, dim3(TPB), 0, st, p);
             else if (kind == kCarryTurn && (p.per_slot || p.sched))
                 hipLaunchKernelGGL((k_carry_turn<R, 8, TPB, true>), grid, dim3(TPB), 0, st, p);
             else if (kind == kCarryTurn) hipLaunchKernelGGL((k_carry_turn<R, 8, TPB, false>), grid, dim3(TPB), 0, st, p);
@@ -1229,6 +1321,7 @@ static void launch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, h
             dim3 grid(p.C / 16, slots);
             if (kind == kInvCarry) hipLaunchKernelGGL((k_inv_carry<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
             else if (kind == kAdjFwd) hipLaunchKernelGGL((k_adjust_fwd<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
+            else if (kind == kAdjOut) hipLaunchKernelGGL((k_adjust_out<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
             else throw std::runtime_error("the one-launch outer pass needs runs of 8 digits");
             return;
         }
@@ -1446,7 +1539,7 @@ static GroupLayout group_layout(const Plan& p, uint32_t cap)
     put(g.car, sizeof(long long) * 3 * g.carry_blocks * cap);
     put(g.runc, sizeof(long long) * 3 * runs * cap);
     put(g.runt, p.nt == 8 ? sizeof(long long) * 3 * runs * cap : 0);
-    put(g.epoch, sizeof(unsigned) * 2 * cap);
+    put(g.epoch, sizeof(unsigned) * 4 * cap);
     put(g.rdig, p.nt ? sizeof(int) * (size_t)p.L * cap : 0);
     put(g.xhi, sizeof(long long) * cap);
     put(g.err, sizeof(unsigned long long) * Group::kChunk * cap);
@@ -1613,7 +1706,7 @@ Group::~Group()
     delete tabs_;
 }
 
-Params Group::params(uint32_t slot0, bool check_active, uint32_t err_col) const
+Params Group::params(uint32_t slot0, bool check_active) const
 {
     Params p;
     p.z = z_;
@@ -1637,7 +1730,7 @@ Params Group::params(uint32_t slot0, bool check_active, uint32_t err_col) const
     p.dig = rdig_;
     p.xhi = xhi_;
     p.xhi_mul = nullptr;
-    p.err = errp_ + err_col;
+    p.err = errp_;
     p.status = status_;
     p.desc = desc_;
     p.state = state_;
@@ -1665,6 +1758,7 @@ Params Group::params(uint32_t slot0, bool check_active, uint32_t err_col) const
 // One kernel kind over `slots` slots starting at p.slot0.
 void Group::launch(int kind, const Params& p, uint32_t slots)
 {
+    launched_ = true;
     switch (kind) {
     case kRow: dispatch_row(p, slots, stream_); break;
     case kCarry:
@@ -1686,7 +1780,7 @@ void Group::launch(int kind, const Params& p, uint32_t slots)
 void Group::set_slot(uint32_t slot, uint32_t n, uint64_t k, uint32_t a, uint64_t total)
 {
     if (slot >= cap_) throw std::invalid_argument("slot out of range");
-    if (n >> plan_.logL < 8 || (double)n / plan_.L > max_bits_per_digit(plan_.logL) + 1.0)
+    if (n >> plan_.logL < 7 || (double)n / plan_.L > max_bits_per_digit(plan_.logL, k) + 1.0)
         throw std::invalid_argument("n does not suit this group's transform length");
     check(hipSetDevice(device_), "hipSetDevice");
     unsigned top = 0;
@@ -1732,7 +1826,7 @@ void Group::upload(uint32_t slot, const std::vector<int32_t>& digits, int64_t xh
     check(hipMemcpyAsync(digits_dev_, digits.data(), sizeof(int32_t) * plan_.L, hipMemcpyHostToDevice, stream_),
           "upload digits");
     check(hipMemcpyAsync(xhi_ + slot, &xh, sizeof(long long), hipMemcpyHostToDevice, stream_), "upload xhi");
-    Params p = params(slot, false, 0);
+    Params p = params(slot, false);
     hipLaunchKernelGGL(k_set_digits, dim3((plan_.L + 255) / 256, 1), dim3(256), 0, stream_, p, digits_dev_);
     check(hipGetLastError(), "k_set_digits");
     check(hipStreamSynchronize(stream_), "upload sync");
@@ -1741,7 +1835,7 @@ void Group::upload(uint32_t slot, const std::vector<int32_t>& digits, int64_t xh
 void Group::download(uint32_t slot, std::vector<int32_t>& digits, int64_t& xhi)
 {
     check(hipSetDevice(device_), "hipSetDevice");
-    Params p = params(slot, false, 0);
+    Params p = params(slot, false);
     hipLaunchKernelGGL(k_get_digits, dim3((plan_.L + 255) / 256, 1), dim3(256), 0, stream_, p, digits_dev_);
     check(hipGetLastError(), "k_get_digits");
     digits.resize(plan_.L);
@@ -1759,10 +1853,16 @@ void Group::download(uint32_t slot, std::vector<int32_t>& digits, int64_t& xhi)
 void Group::enqueue_plain_chunk(int g, bool use_turn)
 {
     const uint32_t s0 = sq0_, ns = sqn_;
-    launch(kFwdCol, params(s0, false, 0), ns);
-    launch(kRow, params(s0, false, 0), ns);
+    if (use_turn) {
+        const size_t count = (size_t)plan_.L * ns;
+        const unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_backup, dim3(blocks), dim3(256), 0, stream_, z_ + (size_t)s0 * plan_.L, zbak_, count,
+                           xhi_ + s0, xbak_, ns);
+    }
+    launch(kFwdCol, params(s0, false), ns);
+    launch(kRow, params(s0, false), ns);
     for (int i = 0; i + 1 < g; ++i) {
-        const Params p = params(s0, false, (uint32_t)i);
+        const Params p = params(s0, false);
         if (use_turn) {
             Params pt = p;
             pt.runc = runt_;
@@ -1778,10 +1878,22 @@ void Group::enqueue_plain_chunk(int g, bool use_turn)
         }
         launch(kRow, p, ns);
     }
-    const Params pl = params(s0, false, (uint32_t)(g - 1));
-    launch(kInvCol, pl, ns);
-    launch(kCarry, pl, ns);
-    launch(kFixup, pl, ns);
+    enqueue_exit(params(s0, false), ns);
+}
+
+// Back to the digit state after a squaring's row pass: the split outer
+// pass's first kernel and k_adjust_out where the split pass exists (2
+// kernels), else inverse column pass, flat carry and fix-up (3).
+void Group::enqueue_exit(const Params& p, uint32_t slots)
+{
+    if (plan_.nt) {
+        launch(kInvCarry, p, slots);
+        launch(kAdjOut, p, slots);
+    } else {
+        launch(kInvCol, p, slots);
+        launch(kCarry, p, slots);
+        launch(kFixup, p, slots);
+    }
 }
 
 // g scheduled steps of every active slot; step i's round-off in err column i.
@@ -1790,7 +1902,7 @@ void Group::enqueue_plain_chunk(int g, bool use_turn)
 void Group::enqueue_step_chunk(int g)
 {
     for (int i = 0; i < g; ++i) {
-        const Params p = params(0, true, (uint32_t)i);
+        const Params p = params(0, true);
         if (plan_.nt) {
             launch(kInvCarry, p, step_n_);
             launch(kAdjFwd, p, step_n_);
@@ -1809,7 +1921,7 @@ void Group::build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, 
 {
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     const uint32_t ns = stepping ? step_n_ : cap_;
-    enqueue_err_clear(ns);
+    enqueue_err_clear(ns, stepping ? 0u : ~0u);
     if (stepping) enqueue_step_chunk(g);
     else enqueue_plain_chunk(g, use_turn);
     enqueue_err_reduce(0, ns);
@@ -1817,12 +1929,19 @@ void Group::build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, 
     check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
 }
 
-// Zero the round-off partials of every slot (a fill kernel, so the chunk
-// can be graph-captured).
-void Group::enqueue_err_clear(uint32_t slots)
+// Zero the round-off partials of slots [0, slots) and set their round-off
+// column to col0 (fill kernels, so the chunk can be graph-captured).  A plain
+// chunk starts at column -1: its entry row pass advances it to 0.
+void Group::enqueue_err_clear(uint32_t slots, unsigned col0)
 {
     const int cnt = (int)((size_t)kChunk * eb_ * slots);
     hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, errp_, cnt);
+    hipLaunchKernelGGL(k_set_col, dim3((slots + 63) / 64), dim3(64), 0, stream_, epoch_, 0u, slots, col0);
+}
+
+void Group::enqueue_set_col(uint32_t slot, unsigned col)
+{
+    hipLaunchKernelGGL(k_set_col, dim3(1), dim3(64), 0, stream_, epoch_, slot, 1u, col);
 }
 
 void Group::enqueue_err_reduce(uint32_t slot0, uint32_t slots)
@@ -1838,7 +1957,7 @@ double Group::read_err(int g, double* trace, double* slot_err, uint32_t trace_sl
     check(hipMemcpyAsync(err_host_.data(), err_, sizeof(unsigned long long) * kChunk * cap_, hipMemcpyDeviceToHost,
                          stream_),
           "err D2H");
-    check(hipStreamSynchronize(stream_), "chunk sync");
+    sync_status("chunk sync");
     double emax = 0.0;
     for (uint32_t s = 0; s < cap_; ++s) {
         double smax = 0.0;
@@ -1857,7 +1976,9 @@ double Group::read_err(int g, double* trace, double* slot_err, uint32_t trace_sl
 double Group::square(uint64_t iters, double* err_trace) { return square_slots(0, cap_, iters, err_trace); }
 
 // One plain chunk of g squarings of slots [sq0_, sq0_ + sqn_): a captured
-// graph for whole chunks, direct launches for a partial one.
+// graph for whole chunks, direct launches for a partial one.  (Partial
+// chunks as a few captured pieces -- entry, blocks of 128, 64, ..., 1
+// steady squarings, exit -- measured slower on MI355X than the launches.)
 void Group::run_plain_chunk(int g, bool use_turn)
 {
     if (g == kChunk) {
@@ -1870,29 +1991,25 @@ void Group::run_plain_chunk(int g, bool use_turn)
             it = plain_graphs_.emplace(key, std::make_pair(ex, gr)).first;
         }
         check(hipGraphLaunch(it->second.first, stream_), "hipGraphLaunch");
+        launched_ = true;
     } else {
-        enqueue_err_clear(cap_);
+        enqueue_err_clear(cap_, ~0u);
         enqueue_plain_chunk(g, use_turn);
         enqueue_err_reduce(0, cap_);
         check(hipGetLastError(), "square launch");
     }
 }
 
-// The digit state of slots [slot0, slot0 + slots) before a one-launch chunk
-// (device-to-device; the backup is allocated on first use).
-void Group::save_start(uint32_t slot0, uint32_t slots)
+// The backup of a one-launch chunk's start (allocated on first use, before
+// any graph that copies into it is captured; the copy itself is the chunk's
+// first kernel).
+void Group::ensure_backup()
 {
-    if (!zbak_) {
-        void* m = nullptr;
-        check(hipMalloc(&m, sizeof(double2) * plan_.L * cap_ + 256 + sizeof(long long) * cap_), "hipMalloc backup");
-        zbak_ = static_cast<double2*>(m);
-        xbak_ = reinterpret_cast<long long*>(static_cast<char*>(m) + sizeof(double2) * plan_.L * cap_ + 256);
-    }
-    check(hipMemcpyAsync(zbak_, z_ + (size_t)slot0 * plan_.L, sizeof(double2) * plan_.L * slots,
-                         hipMemcpyDeviceToDevice, stream_),
-          "backup z");
-    check(hipMemcpyAsync(xbak_, xhi_ + slot0, sizeof(long long) * slots, hipMemcpyDeviceToDevice, stream_),
-          "backup xhi");
+    if (zbak_) return;
+    void* m = nullptr;
+    check(hipMalloc(&m, sizeof(double2) * plan_.L * cap_ + 256 + sizeof(long long) * cap_), "hipMalloc backup");
+    zbak_ = static_cast<double2*>(m);
+    xbak_ = reinterpret_cast<long long*>(static_cast<char*>(m) + sizeof(double2) * plan_.L * cap_ + 256);
 }
 
 void Group::restore_start(uint32_t slot0, uint32_t slots)
@@ -1919,11 +2036,10 @@ double Group::square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, doubl
         // chunk of one squaring has no steady-state squaring)
         TurnToken token(device_, turn() && g >= 2);
         if (token.held) {
-            save_start(slot0, slots);
+            ensure_backup();
             run_plain_chunk(g, true);
             double e = read_err(g, tr, nullptr, slot0);
-            const int st = status();
-            if (st & 2) {
+            if (seen_status_ & 2) {
                 // a hand-off wait gave up (another process's kernels held the
                 // CUs its predecessor needed, or the test hook): the chunk's
                 // result is void; recompute it from its start on the split pass
@@ -1964,20 +2080,19 @@ double Group::mul_slot(uint32_t dst, uint32_t src)
     {
         const int cnt = (int)((size_t)kChunk * eb_);
         hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, errp_ + (size_t)dst * cnt, cnt);
+        enqueue_set_col(dst, 0u);
     }
-    launch(kFwdCol, params(dst, false, 0), 1);
-    launch(kFwdCol, params(src, false, 0), 1);
-    Params p = params(dst, false, 0);
+    launch(kFwdCol, params(dst, false), 1);
+    launch(kFwdCol, params(src, false), 1);
+    Params p = params(dst, false);
     dispatch_row_mul(p, z_ + (size_t)src * plan_.L, xhi_ + src, stream_);
     p.xhi_mul = xhi_ + src;
-    launch(kInvCol, p, 1);
-    launch(kCarry, p, 1);
-    launch(kFixup, p, 1);
+    enqueue_exit(p, 1);
     enqueue_err_reduce(dst, 1);
     check(hipGetLastError(), "mul_slot");
     unsigned long long e = 0;
     check(hipMemcpyAsync(&e, err_ + (size_t)dst * kChunk, sizeof e, hipMemcpyDeviceToHost, stream_), "err D2H");
-    check(hipStreamSynchronize(stream_), "mul_slot sync");
+    sync_status("mul_slot sync");
     double d;
     std::memcpy(&d, &e, sizeof d);
     return d;
@@ -1986,7 +2101,7 @@ double Group::mul_slot(uint32_t dst, uint32_t src)
 void Group::enter(uint32_t slot)
 {
     check(hipSetDevice(device_), "hipSetDevice");
-    const Params p = params(slot, false, 0);
+    const Params p = params(slot, false);
     launch(kFwdCol, p, 1);
     launch(kRow, p, 1);
     check(hipGetLastError(), "enter");
@@ -2014,8 +2129,9 @@ void Group::step_begin(uint32_t g, uint32_t slots)
             it = step_graphs_.emplace(ns, std::make_pair(ex, gr)).first;
         }
         check(hipGraphLaunch(it->second.first, stream_), "hipGraphLaunch");
+        launched_ = true;
     } else {
-        enqueue_err_clear(ns);
+        enqueue_err_clear(ns, 0u);
         enqueue_step_chunk((int)g);
         enqueue_err_reduce(0, ns);
         check(hipGetLastError(), "step launch");
@@ -2054,9 +2170,10 @@ double Group::leave(uint32_t slot)
     {
         const int cnt = (int)((size_t)kChunk * eb_);
         hipLaunchKernelGGL(k_fill_u64, dim3((cnt + 255) / 256), dim3(256), 0, stream_, errp_ + (size_t)slot * cnt, cnt);
+        enqueue_set_col(slot, 0u);
     }
     // the schedule multiplier of the last squaring comes from the slot's state
-    Params p = params(slot, true, 0);
+    Params p = params(slot, true);
     p.sched = 1;
     SlotState st{};
     check(hipMemcpyAsync(&st, state_ + slot, sizeof st, hipMemcpyDeviceToHost, stream_), "state D2H");
@@ -2064,15 +2181,13 @@ double Group::leave(uint32_t slot)
     // the kernels skip inactive slots; the leave sequence runs regardless
     SlotState on{ st.carried, 1, 0 };
     check(hipMemcpyAsync(state_ + slot, &on, sizeof on, hipMemcpyHostToDevice, stream_), "state H2D");
-    launch(kInvCol, p, 1);
-    launch(kCarry, p, 1);
-    launch(kFixup, p, 1);
+    enqueue_exit(p, 1);
     enqueue_err_reduce(slot, 1);
     SlotState fin{ st.carried + 1, 0, 0 };
     check(hipMemcpyAsync(state_ + slot, &fin, sizeof fin, hipMemcpyHostToDevice, stream_), "state H2D");
     unsigned long long e = 0;
     check(hipMemcpyAsync(&e, err_ + (size_t)slot * kChunk, sizeof e, hipMemcpyDeviceToHost, stream_), "err D2H");
-    check(hipStreamSynchronize(stream_), "leave sync");
+    sync_status("leave sync");
     double d;
     std::memcpy(&d, &e, sizeof d);
     return d;
@@ -2093,7 +2208,7 @@ void Group::kernel_times(int reps, double* ms, uint32_t nslots)
     std::vector<int32_t> saved;
     int64_t saved_xhi = 0;
     download(0, saved, saved_xhi);
-    const Params p = params(0, false, 0);
+    const Params p = params(0, false);
     // Real squarings in their steady-state order, an event between every two
     // kernels: each kernel is timed on the data its predecessor just wrote,
     // as it runs inside a squaring chunk.  The first and last kernels of a
@@ -2186,18 +2301,28 @@ void Group::kernel_times(int reps, double* ms, uint32_t nslots)
     upload(0, saved, saved_xhi);
 }
 
+// Enqueues the status word's read-back behind whatever the stream holds,
+// waits for the stream and records the word (one wait for both).
+void Group::sync_status(const char* what)
+{
+    check(hipMemcpyAsync(status_host_, status_, sizeof(int), hipMemcpyDeviceToHost, stream_), "status D2H");
+    check(hipStreamSynchronize(stream_), what);
+    seen_status_ = *status_host_;
+    launched_ = false;
+}
+
 int Group::status()
 {
-    int s = 0;
-    check(hipMemcpyAsync(&s, status_, sizeof(int), hipMemcpyDeviceToHost, stream_), "status D2H");
-    check(hipStreamSynchronize(stream_), "status sync");
-    return s;
+    // the word read at the last wait holds unless kernels ran since
+    if (launched_) sync_status("status sync");
+    return seen_status_;
 }
 
 void Group::clear_status()
 {
     check(hipMemsetAsync(status_, 0, sizeof(int), stream_), "status reset");
     check(hipStreamSynchronize(stream_), "status sync");
+    seen_status_ = 0;
 }
 
 double copy_probe(int device, size_t bytes, int reps)

@@ -101,7 +101,7 @@ LenGroup* group_for(pl_pool* P, uint32_t n, uint64_t k, std::string& why)
     auto it = P->groups.find(pl.logL);
     if (it != P->groups.end()) return it->second.get();
     // n just above the previous length's limit gives the fewest bits per digit
-    const double bmin = pl.logL > 10 ? pleng::max_bits_per_digit(pl.logL - 1) / 2.0 : 8.0;
+    const double bmin = pl.logL > 10 ? pleng::max_bits_per_digit(pl.logL - 1, P->k_max) / 2.0 : 8.0;
     if (!pleng::choose_layout(pl, std::max(8.0, bmin), P->k_max, 100, true, why)) return nullptr;
     const size_t per_slot = pleng::Group::slot_bytes(pl);
     const uint64_t left = P->budget > P->used_bytes ? P->budget - P->used_bytes : 0;

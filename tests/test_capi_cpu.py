@@ -54,8 +54,11 @@ def test_plan_fft_len_fits_and_limits():
         assert 8 <= n / L <= 18.5
     assert core.plan_fft_len(1 << 21, 433917) == 131072
     assert core.plan_fft_len(1 << 24, 11887) == 1 << 20
+    # k near 2^32: the top scalar's share of the round-off caps the digit size
+    assert core.plan_fft_len(500000, 4294967291) == 65536
+    assert core.plan_fft_len(500000, 268435399) == 32768
     with pytest.raises(core.PlError):
-        core.plan_fft_len(4000, 3)       # below 8 bits per digit at the shortest length
+        core.plan_fft_len(4000, 3)       # below 7 bits per digit at the shortest length
     with pytest.raises(core.PlError):
         core.plan_fft_len(10000, 4)      # even k
 
