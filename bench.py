@@ -204,16 +204,17 @@ def plain_per_squaring(core, k, n, iters=1536):
     return dt / iters, L
 
 
-def batch_c4(core, workloads, steps=10000):
+def batch_c4(core, workloads, steps=30000):
     """Config 4: 256 candidates, n uniform in [2^18, 2^20], advanced together
     by the pool (one launch per kernel kind per length group per squaring
     step).  Each test is run for its seed powering plus `steps` squarings;
     tests/hour extrapolates the measured squaring throughput to full tests
     (topbit(k) + n - 1 squarings each).  The timed region includes creating
     the length groups, capturing their graphs, admitting and retiring every
-    candidate: 10^4 steps keep that one-time cost near the share it has in a
-    full test (2.6*10^5..10^6 squarings); at 10^3 steps it halved the
-    measured throughput.  The plain loop runs one candidate at a time: its
+    candidate, and each candidate's first squarings in normal form (while
+    its value is small, a few dozen device products): 3*10^4 steps keep that
+    one-time cost near the share it has in a full test (2.6*10^5..10^6
+    squarings).  The full-length run is batch_c4_full.  The plain loop runs one candidate at a time: its
     per-squaring time is measured per transform length."""
     w = workloads.c4()
     cands = w.candidates
@@ -241,6 +242,58 @@ def batch_c4(core, workloads, steps=10000):
                 plain_us_per_squaring_by_fft_len={str(L): round(v * 1e6, 2) for L, v in sorted(per_len.items())},
                 max_err=max(r["max_err"] for r in res),
                 note=f"each test ran its seed powering + {steps} squarings; tests/hour extrapolates to n-1")
+
+
+def batch_c4_full(core, workloads, out_path=None):
+    """Config 4 at its stated size: the 256 candidates (k odd in [3,2^16), n
+    uniform in [2^18,2^20], PCG64 seed 4) plus the known prime 3*2^303093+1,
+    every one run to its verdict by the pool.  tests/hour counts completed
+    verdicts over the wall time of the whole batch (group creation, graph
+    captures, admissions and retirements included).  The plain loop's figure
+    is its measured steady per-squaring time per transform length times each
+    test's squarings.  Every (k, n, verdict, Res64) goes to out_path."""
+    w = workloads.c4()
+    cands = list(w.candidates) + [(3, 303093)]
+    full = [_topbit(k) + n - 1 for k, n in cands]
+    res = []
+    with core.Pool(k_max=1 << 16) as pool:
+        t0 = time.perf_counter()
+        for i, (k, n) in enumerate(cands):
+            pool.submit(k, n, tag=i)
+        done = [0]
+
+        def progress(r):
+            done[0] += 1
+            if done[0] % 32 == 0:
+                log(f"[bench] c4 full: {done[0]}/{len(cands)} verdicts, {time.perf_counter() - t0:.1f} s")
+        res = pool.drain(steps=64, on_result=progress)
+        t1 = time.perf_counter()
+    by_tag = {r["tag"]: r for r in res}
+    prime_r = by_tag[len(cands) - 1]
+    per_len = {}
+    for (k, n) in cands:
+        L = core.plan_fft_len(n, k)
+        if L not in per_len:
+            per_len[L] = plain_per_squaring(core, k, n)[0]
+    plain_time = sum(per_len[core.plan_fft_len(n, k)] * f for (k, n), f in zip(cands, full))
+    tph = 3600.0 * len(cands) / (t1 - t0)
+    plain_tph = 3600.0 * len(cands) / plain_time
+    if out_path:
+        Path(out_path).write_text(json.dumps([dict(k=k, n=n, verdict=by_tag[i]["verdict"], error=by_tag[i]["error"],
+                                                   res64=format(by_tag[i]["res64"], "016X"),
+                                                   max_err=by_tag[i]["max_err"])
+                                              for i, (k, n) in enumerate(cands)], indent=0))
+    return dict(workload="c4_batch (full): 256 candidates + 3*2^303093+1, every test run to its verdict",
+                tests=len(cands), squarings=sum(full), wall_s=round(t1 - t0, 2),
+                squarings_per_s=round(sum(full) / (t1 - t0), 1), batched_tests_per_hour=round(tph, 1),
+                plain_tests_per_hour=round(plain_tph, 1), speedup_vs_plain=round(tph / plain_tph, 2),
+                plain_us_per_squaring_by_fft_len={str(L): round(v * 1e6, 2) for L, v in sorted(per_len.items())},
+                verdicts={"prime": sum(1 for r in res if r["verdict"] == 1),
+                          "composite": sum(1 for r in res if r["verdict"] == 0),
+                          "void": sum(1 for r in res if r["error"] != 0)},
+                known_prime={"k": 3, "n": 303093, "verdict": prime_r["verdict"],
+                             "res64": format(prime_r["res64"], "016X")},
+                max_err=max(r["max_err"] for r in res))
 
 
 def serve_c5(core, workloads, count=200, scale=100, load=2.0, seed=5):
@@ -309,6 +362,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the n=2^24 line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (config 4) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) leg")
+    ap.add_argument("--c4-full", metavar="OUT.json", default=None,
+                    help="only run config 4 at its stated size (every test to its verdict); per-test results to OUT")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every configuration fits one MI355X and candidates are "
@@ -320,6 +375,9 @@ def main():
     if core.device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
+    if args.c4_full:
+        print(json.dumps(batch_c4_full(core, workloads, args.c4_full)), flush=True)
+        return
 
     (k2, n2), = workloads.c2().candidates
     log(f"[bench] c2: k={k2} n={n2} steps={args.steps} warmup={args.warmup}")

@@ -278,6 +278,18 @@ __device__ __forceinline__ bool fits62(__int128 v)
     return v < ((__int128)1 << 62) && v > -((__int128)1 << 62);
 }
 
+// Balanced split of a 64-bit v by a width b <= 31: v = d + q*2^b with d in
+// [-2^(b-1), 2^(b-1)).  With t = v + 2^(b-1), q = floor(t / 2^b) (an
+// arithmetic shift) and d = (t mod 2^b) - 2^(b-1), which the low 32 bits
+// of t decide: one 64-bit add, one 64-bit shift and 32-bit masking, where a
+// multiply by 2^b would cost a 64-bit product.
+__device__ __forceinline__ int bal_lo(long long v, int b, long long& q)
+{
+    const long long t = v + (long long)(1u << (b - 1));
+    q = t >> b;
+    return (int)((unsigned)t & ((1u << b) - 1u)) - (int)(1u << (b - 1));
+}
+
 template <int W>
 __device__ __forceinline__ void ripple(const Params& p, unsigned m0, __int128 c2in, __int128 extra, long long (&nu)[W])
 {
@@ -293,20 +305,18 @@ __device__ __forceinline__ void ripple(const Params& p, unsigned m0, __int128 c2
         // 64-bit path: |c2in|, |extra| < 2^62 and |km*d| < 2^(32+7+18) keep
         // every sum below 2^63.
         long long rem = (long long)c2in;
-        long long a[W];
+        int d[W];
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-            long long c;
-            const long long d = bal64(rem, bw[j], c);
-            a[j] = -km * d;
-            rem = c;
-        }
+        for (int j = 0; j < W; ++j) d[j] = bal_lo(rem, bw[j], rem);
         long long c = (long long)extra;
+        if (km < (1ll << 31)) {
+            // k*mult fits 32 bits (every pool step, k < 2^31): 32x32 -> 64 products
+            const int nk = -(int)km;
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-            long long cc;
-            nu[j] = bal64(a[j] + c, bw[j], cc);
-            c = cc;
+            for (int j = 0; j < W; ++j) nu[j] = bal_lo((long long)nk * d[j] + c, bw[j], c);
+        } else {
+#pragma unroll
+            for (int j = 0; j < W; ++j) nu[j] = bal_lo(-km * d[j] + c, bw[j], c);
         }
         if (rem != 0 || c != 0) atomicOr(p.status, 1);
         return;
@@ -502,6 +512,8 @@ __global__ __launch_bounds__(TPB) void k_carry(Params pb)
 {
     const Params p = bind(pb, blockIdx.y);
     if (!p.active) return;
+    // round-off column, requested first (its use is the kernel's last store)
+    const unsigned ecol = p.epoch[2] & (Group::kChunk - 1);
     constexpr int SPAN = TPB * RUN;
     __shared__ double2 buf[SPAN + TPB];
     __shared__ long long sq[3][TPB];
@@ -562,7 +574,7 @@ __global__ __launch_bounds__(TPB) void k_carry(Params pb)
         double e = s_err[0];
 #pragma unroll
         for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
-        p.err[(size_t)blockIdx.x * Group::kChunk + (p.epoch[2] & (Group::kChunk - 1))] =
+        p.err[(size_t)blockIdx.x * Group::kChunk + ecol] =
             (unsigned long long)__double_as_longlong(e);
     }
 }
@@ -619,6 +631,7 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
 {
     const Params p = bind(pb, blockIdx.y);
     if (!p.active) return;
+    const unsigned ecol = p.epoch[2] & (Group::kChunk - 1);
     using Lay = LdsLayout<R, NT>;
     constexpr int RPT = (R + TPB - 1) / TPB;
     constexpr int P0 = Radix<R>::first();
@@ -690,7 +703,7 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
         double e = s_err[0];
 #pragma unroll
         for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
-        p.err[(size_t)blockIdx.x * Group::kChunk + (p.epoch[2] & (Group::kChunk - 1))] =
+        p.err[(size_t)blockIdx.x * Group::kChunk + ecol] =
             (unsigned long long)__double_as_longlong(e);
     }
 }
@@ -902,6 +915,7 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     StageTw<R, NT, TPB> t;
     load_stage_tw(t, p.tw);
     const unsigned e = p.epoch[0] & kTagMask;
+    const unsigned ecol = p.epoch[2] & (Group::kChunk - 1);
     if (b == 0 && threadIdx.x == 0) p.epoch[1] = e;
     unsigned ep[RPT];
 #pragma unroll
@@ -1040,7 +1054,7 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
         double e = s_err[0];
 #pragma unroll
         for (int i = 1; i < TPB / 64; ++i) e = fmax(e, s_err[i]);
-        p.err[(size_t)blockIdx.x * Group::kChunk + (p.epoch[2] & (Group::kChunk - 1))] =
+        p.err[(size_t)blockIdx.x * Group::kChunk + ecol] =
             (unsigned long long)__double_as_longlong(e);
     }
     auto ld = [&](int tr, int idx, int, int) { return io[Lay::at(tr, idx)]; };

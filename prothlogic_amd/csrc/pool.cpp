@@ -11,6 +11,7 @@
 // the running batch.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -71,7 +72,9 @@ struct pl_pool {
     uint64_t used_bytes = 0;
     uint32_t max_slots = 0;
     uint64_t k_max = 0;
-    std::map<uint32_t, std::unique_ptr<LenGroup>> groups;  // by log2 L
+    // by (log2 L, digits per carry run): a test joins the group of its
+    // length whose runs absorb the carries at its own digit size
+    std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<LenGroup>> groups;
     std::deque<Request> queue;
     std::vector<pl_pool_result> done;
     clk::time_point t0 = clk::now();
@@ -93,16 +96,29 @@ unsigned topbit64(uint64_t k)
     return t;
 }
 
-// Lowest bits per digit a slot of this length can see, and the layout for it.
+// The group for a test: its transform length, and carry runs of 8 digits
+// when those absorb the carries at the test's own digit size (for any k up
+// to k_max), else 16.  A group holds only tests whose digit size is at
+// least the one its layout was chosen for: runs of 8 at floor(n / L) bits,
+// runs of 16 at the fewest bits the length admits (n just above the
+// previous length's limit).  Steps carry no multiplier (the seed is powered
+// in normal form before a slot starts stepping), so the bound takes 1.
 LenGroup* group_for(pl_pool* P, uint32_t n, uint64_t k, std::string& why)
 {
     pleng::Plan pl;
     if (!pleng::choose_plan(n, k, 0, pl, why)) return nullptr;
-    auto it = P->groups.find(pl.logL);
+    pleng::Plan own = pl;
+    if (!pleng::choose_layout(own, std::floor((double)n / pl.L), P->k_max, 1, true, why)) return nullptr;
+    const uint32_t nt = own.nt == 8 ? 8u : 16u;
+    const auto key = std::make_pair(pl.logL, nt);
+    auto it = P->groups.find(key);
     if (it != P->groups.end()) return it->second.get();
-    // n just above the previous length's limit gives the fewest bits per digit
-    const double bmin = pl.logL > 10 ? pleng::max_bits_per_digit(pl.logL - 1, P->k_max) / 2.0 : 8.0;
-    if (!pleng::choose_layout(pl, std::max(8.0, bmin), P->k_max, 100, true, why)) return nullptr;
+    if (nt == 8) {
+        pl = own;
+    } else {
+        const double bmin = pl.logL > 10 ? pleng::max_bits_per_digit(pl.logL - 1, P->k_max) / 2.0 : 7.0;
+        if (!pleng::choose_layout(pl, std::max(7.0, bmin), P->k_max, 1, true, why)) return nullptr;
+    }
     const size_t per_slot = pleng::Group::slot_bytes(pl);
     const uint64_t left = P->budget > P->used_bytes ? P->budget - P->used_bytes : 0;
     uint32_t cap = (uint32_t)std::min<uint64_t>(P->max_slots, left / (per_slot + 1));
@@ -118,7 +134,7 @@ LenGroup* group_for(pl_pool* P, uint32_t n, uint64_t k, std::string& why)
     lg->scr = plprep::Scratch{ cap, cap + 1, cap + 2 };
     P->used_bytes += lg->g->device_bytes();
     LenGroup* raw = lg.get();
-    P->groups[pl.logL] = std::move(lg);
+    P->groups[key] = std::move(lg);
     return raw;
 }
 
