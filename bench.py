@@ -149,39 +149,124 @@ def measured_traffic(kernel: str, workload: str):
     return None
 
 
-def cpu_baseline(k, n, budget_s=12.0):
-    """The reference CPU engine (oracle/_ref, built from the reference's own
-    sources) timed on this host; the C restatement when that is unavailable."""
+def _cpu_length(k, n):
+    """The reference's starting length for n (n/L <= 18, auto_tune_fft_len),
+    raised until its limb layout holds the residue."""
     sys.path.insert(0, str(ROOT / "oracle"))
-    import numpy as np
-    from oracle import Oracle, RefLib, int_to_limbs
-    import random
+    from oracle import Oracle
     o = Oracle()
     L = 16
     while not o.fits(L, n, k) or n / L > 18.0:
         L *= 2
+    return L
+
+
+def cpu_sample(kind, k, n, seconds):
+    """One process's bounded CPU sample: squarings of a random residue mod
+    k*2^n+1 with fft_square_karatsuba for about `seconds`.  kind: reference
+    (oracle/_ref, the reference's own sources), reference-omp (the same built
+    with -fopenmp; OMP_NUM_THREADS threads) or port (the C restatement)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import random
+    import numpy as np
+    from oracle import Oracle, RefLib, int_to_limbs, _ptr
+    L = _cpu_length(k, n)
+    nl = L // 4
     N = k * (1 << n) + 1
-    x = random.Random(1).randrange(N)
-    v = int_to_limbs(x, L // 4)
-    kind = "reference"
-    try:
-        ref = RefLib()
+    w = np.zeros(nl + 4, dtype=np.uint64)
+    w[:nl] = int_to_limbs(random.Random(1).randrange(N), nl)
+    if kind == "port":
+        oc = Oracle().ctx(L)
+        step = lambda: oc.o.lib.or_square(oc.h, _ptr(w), nl, n, k)
+    else:
+        ref = RefLib(ROOT / "oracle" / "_ref" / ("libprothref_omp.so" if kind == "reference-omp" else "libprothref.so"))
         h = ref.ctx(L)
-        step = lambda w: ref.square_limbs(h, w, L // 4, n, k)
-    except Exception as e:  # not built here, or no AVX-512 on this host
-        log(f"[bench] reference build unavailable ({e}); timing the C restatement")
-        kind = "port"
-        oc = o.ctx(L)
-        step = lambda w: oc.square_limbs(w, L // 4, n, k)
+        step = lambda: ref.lib.ref_fft_square(h, _ptr(w), nl, n, k)
+    step()
     cnt = 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        v = step(v)
+    while time.perf_counter() - t0 < seconds:
+        step()
         cnt += 1
-    dt = time.perf_counter() - t0
-    return dict(value=round(dt / cnt * 1e6, 1), unit="us/squaring", cores=1, kind=kind,
-                sample=f"{cnt} squarings of a random residue mod {k}*2^{n}+1 with fft_square_karatsuba "
-                       f"at L={L} (the shortest length whose limb layout holds the residue), single thread")
+    return dict(count=cnt, seconds=time.perf_counter() - t0, fft_len=L)
+
+
+def _cpu_run(kind, k, n, seconds, procs=1, threads=1):
+    """`procs` concurrent child processes, each running cpu_sample."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    cmd = [sys.executable, str(Path(__file__).resolve()), "--cpu-sample", kind, str(k), str(n), str(seconds)]
+    kids = [subprocess.Popen(cmd, stdout=subprocess.PIPE, env=env) for _ in range(procs)]
+    outs = []
+    for c in kids:
+        out, _ = c.communicate()
+        if c.returncode != 0:
+            raise RuntimeError(f"{kind} sample exited with {c.returncode}")
+        outs.append(json.loads(out.decode().strip().splitlines()[-1]))
+    rate = sum(o["count"] / o["seconds"] for o in outs)  # squarings per second, all processes
+    return dict(us_per_squaring=round(1e6 * procs / rate, 1), squarings_per_s=round(rate, 2),
+                squarings=sum(o["count"] for o in outs), procs=procs, threads=threads, fft_len=outs[0]["fft_len"])
+
+
+def host_cores():
+    """Cores this process may use: the affinity mask, capped by
+    OMP_NUM_THREADS where that is set (the share a batch host grants)."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(k, n, seconds=6.0, kat=(843, 120002)):
+    """The CPU legs, each a bounded sample in child processes:
+      reference-omp  the reference's OpenMP path, one test on every host core
+                     (the headline CPU figure)
+      reference      the reference as published (no -fopenmp), one core
+      port           the C restatement, one core and one process per core
+      workers        tests/hour of the reference's dispatch model (one
+                     single-threaded test per core) at the reference's own
+                     test size n=120002, from squarings/s of every core"""
+    cores = host_cores()
+    legs, errors = {}, {}
+    for name, kind, kk, nn, procs, threads in (
+            ("reference_omp", "reference-omp", k, n, 1, cores),
+            ("reference_1core", "reference", k, n, 1, 1),
+            ("port_1core", "port", k, n, 1, 1),
+            ("port_all_cores", "port", k, n, cores, 1),
+            ("reference_workers_n120002", "reference", kat[0], kat[1], cores, 1)):
+        try:
+            legs[name] = _cpu_run(kind, kk, nn, seconds, procs, threads)
+        except Exception as e:  # not built here, or no AVX-512 on this host
+            errors[name] = str(e)
+    if "reference_workers_n120002" in legs:
+        wk = legs["reference_workers_n120002"]
+        wk["tests_per_hour"] = round(wk["squarings_per_s"] * 3600.0 / (kat[1] + kat[0].bit_length() - 2), 1)
+    if "reference_omp" in legs:
+        head, kind, used = legs["reference_omp"], "reference", cores
+    elif "reference_1core" in legs:
+        head, kind, used = legs["reference_1core"], "reference", 1
+    else:
+        head, kind, used = legs["port_1core"], "port", 1
+    out = dict(value=head["us_per_squaring"], unit="us/squaring", cores=used, kind=kind,
+               sample=(f"{head['squarings']} squarings of a random residue mod {k}*2^{n}+1 with "
+                       f"fft_square_karatsuba at L={head['fft_len']} (the reference's starting length, raised "
+                       f"until its limb layout holds the residue), {used} thread(s), ~{seconds:g} s"),
+               cpu_model=cpu_model(), host_cores=cores, legs=legs)
+    if "reference_1core" in legs and "port_1core" in legs:
+        out["port_over_reference_1core"] = round(legs["port_1core"]["us_per_squaring"]
+                                                 / legs["reference_1core"]["us_per_squaring"], 2)
+    if errors:
+        out["errors"] = errors
+    return out
 
 
 def _topbit(k: int) -> int:
@@ -364,14 +449,34 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) leg")
     ap.add_argument("--c4-full", metavar="OUT.json", default=None,
                     help="only run config 4 at its stated size (every test to its verdict); per-test results to OUT")
+    ap.add_argument("--cpu-only", action="store_true", help="only the CPU legs (no GPU needed)")
+    ap.add_argument("--cpu-sample", nargs=4, metavar=("KIND", "K", "N", "SECONDS"), help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_sample:
+        kind, k, n, sec = args.cpu_sample
+        print(json.dumps(cpu_sample(kind, int(k), int(n), float(sec))), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every configuration fits one MI355X and candidates are "
               "independent (they shard by candidate with no communication); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
-    from prothlogic_amd import core, workloads
+    from prothlogic_amd import workloads
+    (k2, n2), = workloads.c2().candidates
+    cpu = None
+    if args.cpu_only or not args.no_cpu:
+        # before the GPU is initialised: the legs run in child processes
+        try:
+            cpu = cpu_baseline(k2, n2)
+            log(f"[bench] cpu baseline: {cpu}")
+        except Exception as e:
+            log(f"[bench] cpu baseline failed: {e}")
+        if args.cpu_only:
+            print(json.dumps(cpu), flush=True)
+            return
+
+    from prothlogic_amd import core
     if core.device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
@@ -379,7 +484,6 @@ def main():
         print(json.dumps(batch_c4_full(core, workloads, args.c4_full)), flush=True)
         return
 
-    (k2, n2), = workloads.c2().candidates
     log(f"[bench] c2: k={k2} n={n2} steps={args.steps} warmup={args.warmup}")
     # Res64 checked after 10^4 squarings from the seed (the count the GMP
     # fixture holds) in an untimed second run, whatever --steps is
@@ -421,14 +525,6 @@ def main():
         log("[bench] c5 serving")
         extra["serve_c5"] = serve_c5(core, workloads)
         log(f"[bench] c5: {extra['serve_c5']}")
-
-    cpu = None
-    if not args.no_cpu:
-        try:
-            cpu = cpu_baseline(k2, n2)
-            log(f"[bench] cpu baseline: {cpu}")
-        except Exception as e:
-            log(f"[bench] cpu baseline failed: {e}")
 
     line = {
         "metric": METRIC,
