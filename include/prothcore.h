@@ -37,7 +37,7 @@ extern "C" {
 #define PL_ERR_STATE (-4)    /* device carry did not settle (internal bug)  */
 #define PL_ERR_IO (-5)       /* checkpoint file could not be read/written   */
 
-#define PL_ABI_VERSION 4
+#define PL_ABI_VERSION 5
 
 typedef struct pl_ctx pl_ctx;
 
@@ -54,6 +54,10 @@ typedef struct pl_info {
                               launches (PL_CARRY_TURN=0), 5 unfused          */
     uint32_t handoff_fallbacks; /* chunks recomputed on the two-launch pass
                               after a carry hand-off wait gave up            */
+    uint32_t row_radix;    /* largest radix of the steady-state row pass    */
+    uint32_t col_radix;    /* largest radix of the steady-state outer pass  */
+    uint32_t run;          /* digits per carry run of the outer pass (8 or
+                              16; 0 unfused)                                 */
 } pl_info;
 
 /* Thread-local description of the last error (never NULL). */
@@ -76,6 +80,44 @@ int pl_plan_fft_len(uint32_t n, uint64_t k, uint32_t* fft_len);
  * its probe round-off, *us_per_squaring (nullable) its measured time. */
 int pl_autotune_fft_len(uint32_t n, uint64_t k, int device, double err_guard, uint32_t* fft_len, double* max_err,
                         double* us_per_squaring);
+
+/* Launch-plan autotuner (no counterpart in the reference, whose one length
+ * test is P/dcore.cpp:20-81).  For a length the engine can run several
+ * shapes L = R*C, row and column radices 8 or 16, carry runs of 8 or 16
+ * digits and the outer pass as one launch or two.  A tuning run times them
+ * on the device (coordinate search from the built-in rule: shape, row radix,
+ * column radix, runs, outer-pass form), each from the same random residues
+ * for the same number of whole 256-squaring chunks, and keeps the fastest
+ * whose residue after those squarings equals the rule plan's and whose
+ * round-off stays within 0.30.  The choice goes into a process-wide table
+ * that contexts (pl_ctx_create) and pool groups created later take, keyed by
+ * device, length, carry layout, slot-count class (1, 2-8, 9-64, more) and
+ * outer-pass constraint.  pl_autotune_fft_len also tunes the length it
+ * picks.  Each trial is reported (trials may be NULL; at most max_trials are
+ * written, *n_trials gets the count); *best (nullable) is the choice. */
+typedef struct pl_tune_trial {
+    uint32_t rows, cols;     /* R x C                                          */
+    uint32_t row_radix;      /* 8 or 16                                        */
+    uint32_t col_radix;      /* 8 or 16                                        */
+    uint32_t run;            /* digits per carry run (8, 16; 0 unfused)        */
+    uint32_t one_launch;     /* outer pass as one launch                       */
+    double us_per_squaring;  /* device time per squaring of every slot         */
+    double max_err;          /* round-off of the timed squarings               */
+    int32_t exact;           /* residue equal to the rule plan's               */
+    int32_t chosen;          /* the trial the table now holds                  */
+} pl_tune_trial;
+/* Tunes the plan of a single context for (k, n) (pl_ctx_create's, slots 1). */
+int pl_autotune_plan(int device, uint64_t k, uint32_t n, pl_tune_trial* trials, uint32_t max_trials,
+                     uint32_t* n_trials, pl_tune_trial* best);
+/* Puts `choice` (rows, radices, run, one_launch) into the table for the
+ * plan a single context for (k, n) would take; PL_ERR_ARG when the kernels
+ * have no such plan for that length or carry layout. */
+int pl_tune_pin(int device, uint64_t k, uint32_t n, const pl_tune_trial* choice);
+/* Empties the table (every later plan takes the built-in rule). */
+int pl_tune_clear(void);
+/* Writes / merges the table as text (one entry a line). */
+int pl_tune_save(const char* path);
+int pl_tune_load(const char* path);
 
 /* Context create/destroy (replaces fft_init / fft_destroy,
  * P/fft_engine.h:10-11).  fft_len = 0 picks pl_plan_fft_len's length.
@@ -297,10 +339,12 @@ typedef struct pl_pool_result {
 
 /* hbm_budget_bytes = 0: 64 GiB; max_slots_per_length = 0: 256;
  * k_max = 0: 2^32-1.  Pool groups overlap on their own streams and always
- * take the two-launch outer pass (no workgroup waits on another).  A length
- * group's carry runs must absorb carries at the narrowest digits the length
- * admits (about 9 bits): k_max chooses runs of 8 digits (below 2^12 at
- * L <= 2^15, 2^11 at L = 2^16..2^17) or else 16. */
+ * take the two-launch outer pass (no workgroup waits on another).  Each
+ * length keeps up to two groups: carry runs of 8 digits for tests whose own
+ * digit size lets 8 digits absorb the carries of any k up to k_max, and runs
+ * of 16 for the rest (sized for the narrowest digits the length admits).  A
+ * group takes the tuned plan for its key when pl_pool_autotune (or a table
+ * loaded with pl_tune_load) recorded one before the group was created. */
 int pl_pool_create(int device, uint64_t hbm_budget_bytes, uint32_t max_slots_per_length, uint64_t k_max,
                    pl_pool** out);
 void pl_pool_destroy(pl_pool* pool);
@@ -314,6 +358,11 @@ int pl_pool_submit(pl_pool* pool, uint64_t k, uint32_t n, uint64_t tag, uint64_t
  * restarted from its seed (or reported void, pl_pool_result.error). */
 int pl_pool_run(pl_pool* pool, uint32_t steps, pl_pool_result* out, size_t cap, size_t* n_out);
 int pl_pool_status(pl_pool* pool, size_t* queued, size_t* active, size_t* finished_unread, uint64_t* hbm_bytes);
+/* Tunes the plan of the length group a candidate (k, n) would join, with
+ * the slot count that group would get from this pool's budget (see
+ * pl_autotune_plan).  A group that already exists keeps its plan. */
+int pl_pool_autotune(pl_pool* pool, uint64_t k, uint32_t n, pl_tune_trial* trials, uint32_t max_trials,
+                     uint32_t* n_trials, pl_tune_trial* best);
 const char* pl_pool_last_error(void);
 
 #ifdef __cplusplus

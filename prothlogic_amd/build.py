@@ -30,7 +30,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
              "-Xclang", "-packed-fp32-ops", "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 
 CORE_SRCS = [CSRC / "engine.hip", CSRC / "capi.cpp", CSRC / "bignum.cpp", CSRC / "prewrap.cpp", CSRC / "pool.cpp",
-             CSRC / "sieve.hip", CSRC / "noalloc.hip"]
+             CSRC / "tune.cpp", CSRC / "sieve.hip", CSRC / "noalloc.hip"]
 HOST_SRCS = [HOST / "fft_engine.cpp", HOST / "proth_logic.cpp", HOST / "checkpoint_manager.cpp",
              HOST / "worker.cpp", HOST / "autotune.cpp", HOST / "presieve_core.cpp"]
 

@@ -33,7 +33,18 @@ class PlInfo(ctypes.Structure):
     _fields_ = [("k", ctypes.c_uint64), ("n", ctypes.c_uint32), ("fft_len", ctypes.c_uint32),
                 ("rows", ctypes.c_uint32), ("cols", ctypes.c_uint32), ("bits_per_digit", ctypes.c_double),
                 ("device_bytes", ctypes.c_uint64), ("passes", ctypes.c_uint32),
-                ("handoff_fallbacks", ctypes.c_uint32)]
+                ("handoff_fallbacks", ctypes.c_uint32), ("row_radix", ctypes.c_uint32),
+                ("col_radix", ctypes.c_uint32), ("run", ctypes.c_uint32)]
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_uint32), ("cols", ctypes.c_uint32), ("row_radix", ctypes.c_uint32),
+                ("col_radix", ctypes.c_uint32), ("run", ctypes.c_uint32), ("one_launch", ctypes.c_uint32),
+                ("us_per_squaring", ctypes.c_double), ("max_err", ctypes.c_double), ("exact", ctypes.c_int32),
+                ("chosen", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double)
@@ -94,6 +105,14 @@ def lib() -> ctypes.CDLL:
         "pl_kernel_times": ([vp, i32, _dp, _u64p], i32),
         "pl_copy_probe": ([i32, u64, i32, _dp], i32),
         "pl_autotune_fft_len": ([u32, u64, i32, dbl, ctypes.POINTER(u32), _dp, _dp], i32),
+        "pl_autotune_plan": ([i32, u64, u32, ctypes.POINTER(TuneTrial), u32, ctypes.POINTER(u32),
+                              ctypes.POINTER(TuneTrial)], i32),
+        "pl_tune_pin": ([i32, u64, u32, ctypes.POINTER(TuneTrial)], i32),
+        "pl_tune_clear": ([], i32),
+        "pl_tune_save": ([ctypes.c_char_p], i32),
+        "pl_tune_load": ([ctypes.c_char_p], i32),
+        "pl_pool_autotune": ([vp, u64, u32, ctypes.POINTER(TuneTrial), u32, ctypes.POINTER(u32),
+                              ctypes.POINTER(TuneTrial)], i32),
         "pl_sieve_range": ([u64, u64, u32, u64, i32, _u64p, sz, ctypes.POINTER(sz)], i32),
         "pl_sieve_list": ([_u64p, sz, u32, u64, u64, i32, ctypes.POINTER(ctypes.c_uint8)], i32),
         "pl_sieve_last_error": ([], ctypes.c_char_p),
@@ -312,6 +331,40 @@ def autotune_fft_len(n: int, k: int, device: int = 0, err_guard: float = 0.0) ->
     return dict(fft_len=L.value, max_err=e.value, us_per_squaring=us.value)
 
 
+def _tune_out(call):
+    trials = (TuneTrial * 64)()
+    nt = ctypes.c_uint32(0)
+    best = TuneTrial()
+    call(trials, 64, ctypes.byref(nt), ctypes.byref(best))
+    return dict(best=best.as_dict(), trials=[trials[i].as_dict() for i in range(min(nt.value, 64))])
+
+
+def autotune_plan(k: int, n: int, device: int = 0) -> dict:
+    """Times the launch plans of a single context for (k, n) and records the
+    fastest exact one (pl_autotune_plan)."""
+    return _tune_out(lambda t, m, c, b: _check(lib().pl_autotune_plan(device, k, n, t, m, c, b)))
+
+
+def tune_pin(k: int, n: int, rows: int, row_radix: int = 8, col_radix: int = 8, run: int = 8,
+             one_launch: bool = True, device: int = 0):
+    """Records a plan choice for single contexts of (k, n) (pl_tune_pin)."""
+    c = TuneTrial(rows=rows, cols=0, row_radix=row_radix, col_radix=col_radix, run=run,
+                  one_launch=1 if one_launch else 0)
+    _check(lib().pl_tune_pin(device, k, n, ctypes.byref(c)))
+
+
+def tune_clear():
+    _check(lib().pl_tune_clear())
+
+
+def tune_save(path: str):
+    _check(lib().pl_tune_save(str(path).encode()))
+
+
+def tune_load(path: str):
+    _check(lib().pl_tune_load(str(path).encode()))
+
+
 def save_checkpoint(path: str, k: int, n: int, base: int, next_iter: int, x: int, nl: int):
     _check(lib().pl_save_checkpoint(path.encode(), k, n, base, next_iter, _limbs(x, nl), nl))
 
@@ -356,6 +409,11 @@ class Pool:
 
     def submit(self, k: int, n: int, tag: int = 0, stop_after: int = 0):
         _pcheck(lib().pl_pool_submit(self.h, k, n, tag, stop_after))
+
+    def autotune(self, k: int, n: int) -> dict:
+        """Tunes the plan of the group (k, n) would join (pl_pool_autotune);
+        call before that group is created."""
+        return _tune_out(lambda t, m, c, b: _pcheck(lib().pl_pool_autotune(self.h, k, n, t, m, c, b)))
 
     def run(self, steps: int = 64, cap: int = 4096):
         buf = (PoolResult * cap)()

@@ -17,6 +17,7 @@
 #include "bignum.h"
 #include "engine.h"
 #include "prewrap.h"
+#include "tune.h"
 
 namespace {
 thread_local std::string g_err = "no error";
@@ -186,6 +187,31 @@ struct pl_ctx {
 
 namespace {
 
+// The built-in plan of a single context: length rule (or fft_len), then the
+// carry layout for the test's own digits and k.
+bool rule_plan(uint32_t n, uint64_t k, uint32_t fft_len, pleng::Plan& p, std::string& why)
+{
+    if (!pleng::choose_plan(n, k, fft_len, p, why)) return false;
+    const char* fz = std::getenv("PL_FUSED");
+    const bool allow_split = !(fz && fz[0] == '0');
+    return pleng::choose_layout(p, (double)n / p.L, k, 1, allow_split, why);
+}
+
+int check_device(int device)
+{
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return fail(PL_ERR_HIP, "no HIP device visible");
+    if (device < 0 || device >= nd) return fail(PL_ERR_ARG, "device index out of range");
+    return PL_OK;
+}
+
+uint32_t log2_of(uint32_t x)
+{
+    uint32_t l = 0;
+    while ((1u << l) < x) ++l;
+    return l;
+}
+
 int create_ctx(uint64_t k, uint32_t n, uint32_t fft_len, int device, uint32_t capacity, pl_ctx** out)
 {
     if (!out) return fail(PL_ERR_ARG, "out is NULL");
@@ -194,15 +220,14 @@ int create_ctx(uint64_t k, uint32_t n, uint32_t fft_len, int device, uint32_t ca
         auto c = std::make_unique<pl_ctx>();
         std::string why;
         if (!pleng::choose_plan(n, k, fft_len, c->plan, why)) return fail(PL_ERR_ARG, why);
-        int nd = 0;
-        if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return fail(PL_ERR_HIP, "no HIP device visible");
-        if (device < 0 || device >= nd) return fail(PL_ERR_ARG, "device index out of range");
+        if (int rc = check_device(device)) return rc;
         c->device = device;
         c->M = std::make_unique<plbn::Modulus>(n, k);
-        const char* fz = std::getenv("PL_FUSED");
-        const bool allow_split = !(fz && fz[0] == '0');
-        if (!pleng::choose_layout(c->plan, (double)n / c->plan.L, k, 1, allow_split, why)) return fail(PL_ERR_ARG, why);
+        if (!rule_plan(n, k, fft_len, c->plan, why)) return fail(PL_ERR_ARG, why);
+        bool turn = true;
+        const bool tuned = pltune::apply(device, c->plan, 1, false, &turn);
         c->eng = std::make_unique<pleng::Group>(c->plan, capacity + 3, device);
+        if (tuned && !turn) c->eng->no_turn();
         c->scr = plprep::Scratch{ capacity, capacity + 1, capacity + 2 };
         *out = c.release();
         return PL_OK;
@@ -255,6 +280,9 @@ int pl_ctx_info(const pl_ctx* ctx, pl_info* out)
     out->device_bytes = ctx->eng->device_bytes();
     out->passes = (uint32_t)ctx->eng->launches_per_squaring();
     out->handoff_fallbacks = ctx->eng->handoff_fallbacks();
+    out->row_radix = ctx->plan.row_radix;
+    out->col_radix = ctx->plan.col_radix;
+    out->run = ctx->plan.nt;
     return PL_OK;
 }
 
@@ -415,9 +443,85 @@ int pl_autotune_fft_len(uint32_t n, uint64_t k, int device, double err_guard, ui
         }
     }
     if (!best) return fail(PL_ERR_ROUNDOFF, last_why);
+    // then shape, radices and outer pass at that length
+    rc = guarded([&] {
+        pleng::Plan p;
+        std::string why;
+        if (!rule_plan(n, k, best, p, why)) return fail(PL_ERR_ARG, why);
+        pltune::Trial bt;
+        pltune::tune(device, p, 1, false, bt);
+        best_us = std::min(best_us, bt.us);
+        return PL_OK;
+    });
+    if (rc) return rc;
     *fft_len = best;
     if (max_err) *max_err = best_err;
     if (us_per_squaring) *us_per_squaring = best_us;
+    return PL_OK;
+}
+
+int pl_autotune_plan(int device, uint64_t k, uint32_t n, pl_tune_trial* trials, uint32_t max_trials,
+                     uint32_t* n_trials, pl_tune_trial* best)
+{
+    if (int rc = check_device(device)) return rc;
+    return guarded([&] {
+        pleng::Plan p;
+        std::string why;
+        if (!rule_plan(n, k, 0, p, why)) return fail(PL_ERR_ARG, why);
+        pltune::Trial b;
+        const auto ts = pltune::tune(device, p, 1, false, b);
+        pltune::report(ts, b, trials, max_trials, n_trials, best);
+        return PL_OK;
+    });
+}
+
+int pl_tune_pin(int device, uint64_t k, uint32_t n, const pl_tune_trial* choice)
+{
+    if (!choice) return fail(PL_ERR_ARG, "choice is NULL");
+    if (device < 0) return fail(PL_ERR_ARG, "device index out of range");
+    pleng::Plan p;
+    std::string why;
+    if (!rule_plan(n, k, 0, p, why)) return fail(PL_ERR_ARG, why);
+    pltune::Choice c;
+    if (choice->rows == 0 || (choice->rows & (choice->rows - 1)) || choice->rows >= p.L)
+        return fail(PL_ERR_ARG, "rows must be a power of two below fft_len");
+    c.logR = log2_of(choice->rows);
+    if (choice->cols && choice->cols * choice->rows != p.L) return fail(PL_ERR_ARG, "rows * cols must equal fft_len");
+    c.row_radix = choice->row_radix;
+    c.col_radix = choice->col_radix;
+    c.nt = choice->run;
+    c.turn = choice->one_launch != 0;
+    if ((c.row_radix != 8 && c.row_radix != 16) || (c.col_radix != 8 && c.col_radix != 16))
+        return fail(PL_ERR_ARG, "radices must be 8 or 16");
+    if (!(c.nt == p.nt || (p.nt == 8 && c.nt == 16)))
+        return fail(PL_ERR_ARG, "run must be the carry layout's (" + std::to_string(p.nt) + ") or 16 in place of 8");
+    if (c.turn && c.nt != 8) return fail(PL_ERR_ARG, "the one-launch outer pass needs runs of 8 digits");
+    const auto opts = pleng::shape_options(p.logL, c.nt != 0);
+    if (std::find(opts.begin(), opts.end(), c.logR) == opts.end())
+        return fail(PL_ERR_ARG, "no kernels for this shape with this outer pass");
+    pltune::set(pltune::key_for(device, p, 1, false), c);
+    return PL_OK;
+}
+
+int pl_tune_clear(void)
+{
+    pltune::clear();
+    return PL_OK;
+}
+
+int pl_tune_save(const char* path)
+{
+    if (!path) return fail(PL_ERR_ARG, "path is NULL");
+    std::string why;
+    if (!pltune::save(path, why)) return fail(PL_ERR_IO, why);
+    return PL_OK;
+}
+
+int pl_tune_load(const char* path)
+{
+    if (!path) return fail(PL_ERR_ARG, "path is NULL");
+    std::string why;
+    if (!pltune::load(path, why)) return fail(PL_ERR_IO, why);
     return PL_OK;
 }
 

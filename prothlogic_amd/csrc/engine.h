@@ -26,7 +26,17 @@ struct Plan {
     uint32_t s = 0;       // log2 of the low half of the two-level tables
     uint32_t nt = 0;      // split outer pass with runs of nt digits (8 or 16), 0 = unfused
     uint32_t run = 16;    // digits per thread in the flat carry kernel (8 or 16)
+    // largest radix of the steady-state row pass and of the steady-state
+    // outer (column) pass, 8 or 16; the entry, exit and product passes use 8
+    uint32_t row_radix = 8, col_radix = 8;
 };
+
+// Shapes a plan's length allows: log2 R for L = R*C with R in [32, 1024],
+// C in [32, 4096] (the kernels' instantiated lengths), and R in [64, 512]
+// where the split outer pass is wanted.
+std::vector<uint32_t> shape_options(uint32_t logL, bool split);
+// Sets R = 2^logR, C = L/R (and the two-level table split) on a plan.
+void set_shape(Plan& p, uint32_t logR);
 
 // Smallest power of two L in [2^10, 2^22] with n/L <= max_bits_per_digit(L, k)
 // and n/L >= 7.  force_L, when nonzero, is checked and used instead.

@@ -434,7 +434,7 @@ __global__ __launch_bounds__(TPB) void k_inv_col(Params pb)
 // radix 8) the forward last stage, the pointwise step and the inverse first
 // stage run in registers.  The stage twiddles are read once and serve both
 // directions.
-template <int C, int NT, int TPB, bool SLOTS>
+template <int C, int NT, int TPB, bool SLOTS, int RADIX>
 __global__ __launch_bounds__(TPB) void k_row(Params pb)
 {
     const Params p = bind<SLOTS>(pb, blockIdx.y);
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(TPB) void k_row(Params pb)
     // that is 128 KB, which one workgroup per CU has to itself anyway
     __shared__ double2 lds[2][LdsLayout<C, NT>::kSlots];
     double2* const buf[2] = { lds[0], lds[1] };
-    StageTw<C, NT, TPB, PL_ROW_RADIX> t;
+    StageTw<C, NT, TPB, RADIX> t;
     load_stage_tw(t, p.tw);
     const size_t base = (size_t)blockIdx.x * NT * C;
     const double2* __restrict__ z = p.z;
@@ -454,10 +454,10 @@ __global__ __launch_bounds__(TPB) void k_row(Params pb)
     // waited for there, in front of block 0's row loads).
     const unsigned last = __hip_atomic_load(p.epoch + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned col = __hip_atomic_load(p.epoch + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    dft_square_pass_pp<C, NT, TPB, PL_ROW_RADIX>(
+    dft_square_pass_pp<C, NT, TPB, RADIX>(
         buf, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
         [&](double2 X) { return cmul(X, make_double2(X.x, X.y + x2)); },
-        [&](int tr, int idx, double2 v, int, int) { p.z[base + (size_t)tr * C + idx] = v; });
+        [&](int tr, int idx, double2 v, int, int) { store_stream(&p.z[base + (size_t)tr * C + idx], v); });
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         p.epoch[0] = last + 1u;
         p.epoch[2] = col + 1u;  // the next outer pass carries the next squaring
@@ -496,7 +496,7 @@ __global__ __launch_bounds__(TPB) void k_row_mul(Params pb, const double2* __res
     __syncthreads();
     dft<C, NT, TPB, true, true>(
         lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; },
-        [&](int tr, int idx, double2 v, int, int) { p.z[base + (size_t)tr * C + idx] = v; });
+        [&](int tr, int idx, double2 v, int, int) { store_stream(&p.z[base + (size_t)tr * C + idx], v); });
 }
 
 // ------------------------------------------------------------- flat carry
@@ -626,7 +626,7 @@ __global__ __launch_bounds__(64) void k_fixup(Params pb, unsigned nblocks)
 // run; writes the runs' digits (before their carry-in) and carry-outs.  All
 // table values the block needs (stage twiddles, four-step twiddles, row and
 // column weights) are requested before the transform, beside the data.
-template <int R, int NT, int TPB>
+template <int R, int NT, int TPB, int MAXR>
 __global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
 {
     const Params p = bind(pb, blockIdx.y);
@@ -634,13 +634,13 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
     const unsigned ecol = p.epoch[2] & (Group::kChunk - 1);
     using Lay = LdsLayout<R, NT>;
     constexpr int RPT = (R + TPB - 1) / TPB;
-    constexpr int P0 = Radix<R>::first();
+    constexpr int P0 = Radix<R, MAXR>::first();
     __shared__ double2 lds[Lay::kSlots];
     __shared__ double s_err[TPB / 64];
     const unsigned b = blockIdx.x;
     const unsigned c0 = b * NT;
     const double2* __restrict__ z = p.z;
-    StageTw<R, NT, TPB> t;
+    StageTw<R, NT, TPB, MAXR> t;
     load_stage_tw(t, p.tw);
     ColTw<R, NT, TPB, P0> ct;
     ct.load(p, c0 + threadIdx.x % NT);
@@ -712,21 +712,21 @@ __global__ __launch_bounds__(TPB) void k_inv_carry(Params pb)
 // digits are weighted, and the block runs its forward column DFT.  Block 0
 // also replaces X_hi with the top run's third carry (it is the only reader
 // of the old value: X_hi^2 enters run (0, 0)).
-template <int R, int NT, int TPB>
+template <int R, int NT, int TPB, int MAXR>
 __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
 {
     const Params p = bind(pb, blockIdx.y);
     if (!p.active) return;
     using Lay = LdsLayout<R, NT>;
     constexpr int RPT = (R + TPB - 1) / TPB;
-    constexpr int PL = Radix<R>::last();
+    constexpr int PL = Radix<R, MAXR>::last();
     __shared__ double2 lds[Lay::kSlots];
     const unsigned nb = p.C / NT;
     const unsigned b = blockIdx.x;
     const unsigned c0 = b * NT;
     const long long k = p.k;
     const long long* top = p.runc + 3 * ((size_t)(nb - 1) * R + (R - 1));
-    StageTw<R, NT, TPB> t;
+    StageTw<R, NT, TPB, MAXR> t;
     load_stage_tw(t, p.tw);
     ColTw<R, NT, TPB, PL> ct;
     ct.load(p, c0 + threadIdx.x % NT);
@@ -778,7 +778,7 @@ __global__ __launch_bounds__(TPB) void k_adjust_fwd(Params pb)
     dft<R, NT, TPB, false, true, false>(
         lds, t, [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; },
         [&](int tr, int idx, double2 v, int u, int r) {
-            p.z[(size_t)idx * p.C + c0 + tr] = cmul(v, ct.w[u][r]);
+            store_stream(&p.z[(size_t)idx * p.C + c0 + tr], cmul(v, ct.w[u][r]));
         });
 }
 
@@ -889,19 +889,19 @@ __device__ __forceinline__ unsigned long long tag_word(double q, unsigned ep)
 __device__ __forceinline__ long long untag(unsigned long long w) { return (long long)w >> kTagBits; }
 __device__ __forceinline__ unsigned tag_of(unsigned long long w) { return (unsigned)w & kTagMask; }
 
-template <int R, int NT, int TPB, bool SLOTS>
+template <int R, int NT, int TPB, bool SLOTS, int MAXR>
 __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
 {
     const Params p = bind<SLOTS>(pb, blockIdx.y);
     if (!p.active) return;
     using Lay = LdsLayout<R, NT>;
     constexpr int RPT = (R + TPB - 1) / TPB;
-    constexpr int P0 = Radix<R>::first();
-    constexpr int PL = Radix<R>::last();
+    constexpr int P0 = Radix<R, MAXR>::first();
+    constexpr int PL = Radix<R, MAXR>::last();
     // Two LDS buffers (ping-pong stages, half the barriers) while two
     // workgroups still fit a CU; one buffer for R*NT > 2048 points.
     constexpr bool PP = Lay::kSlots <= 2048;
-    constexpr int IO = pp_out<R, PL_MAX_RADIX, 0>();  // buffer the inverse DFT leaves the columns in
+    constexpr int IO = pp_out<R, MAXR, 0>();  // buffer the inverse DFT leaves the columns in
     __shared__ double2 lds[PP ? 2 : 1][Lay::kSlots];
     double2* const buf[2] = { lds[0], lds[PP ? 1 : 0] };
     double2* const io = buf[PP ? IO : 0];
@@ -912,7 +912,7 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     const double2* __restrict__ z = p.z;
     unsigned long long* const rw = reinterpret_cast<unsigned long long*>(p.runc);
     const size_t runs = (size_t)nb * R;
-    StageTw<R, NT, TPB> t;
+    StageTw<R, NT, TPB, MAXR> t;
     load_stage_tw(t, p.tw);
     const unsigned e = p.epoch[0] & kTagMask;
     const unsigned ecol = p.epoch[2] & (Group::kChunk - 1);
@@ -1058,7 +1058,9 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
             (unsigned long long)__double_as_longlong(e);
     }
     auto ld = [&](int tr, int idx, int, int) { return io[Lay::at(tr, idx)]; };
-    auto st = [&](int tr, int idx, double2 v, int u, int r) { p.z[(size_t)idx * p.C + c0 + tr] = cmul(v, ct.w[u][r]); };
+    auto st = [&](int tr, int idx, double2 v, int u, int r) {
+        store_stream(&p.z[(size_t)idx * p.C + c0 + tr], cmul(v, ct.w[u][r]));
+    };
     if constexpr (PP) dft_pp<R, NT, TPB, false, 1 - IO>(buf, t, ld, st);
     else dft<R, NT, TPB, false, true, false>(io, t, ld, st);
 }
@@ -1262,6 +1264,24 @@ bool choose_plan(uint32_t n, uint64_t k, uint32_t force_L, Plan& out, std::strin
     return true;
 }
 
+std::vector<uint32_t> shape_options(uint32_t logL, bool split)
+{
+    std::vector<uint32_t> out;
+    for (uint32_t lr = 5; lr <= 10 && lr < logL; ++lr) {
+        const uint32_t lc = logL - lr;
+        if (lc < 5 || lc > 12) continue;
+        if (split && (lr < 6 || lr > 9)) continue;
+        out.push_back(lr);
+    }
+    return out;
+}
+
+void set_shape(Plan& p, uint32_t logR)
+{
+    p.R = 1u << logR;
+    p.C = p.L / p.R;
+}
+
 bool choose_layout(Plan& p, double bmin, uint64_t kmax, uint32_t mult_max, bool allow_split, std::string& why)
 {
     const int w = ripple_digits(p.logL, bmin, kmax, mult_max);
@@ -1281,18 +1301,18 @@ struct ColCfg {
     static constexpr int TPB = tpb_for(NT * R);
 };
 
-template <int R, int NT>
+template <int R, int NT, int MAXR = 8>
 struct SplitCfg {
-    static constexpr int TPB = tpb_for(NT * R);
+    static constexpr int TPB = tpb_for(NT * R, MAXR);
 };
 
 // Row blocks of 512 points (NT = 512 / C rows for short rows): more, smaller
 // workgroups than 1024-point blocks, 0.6-0.9 us per squaring faster at
 // C <= 512 on MI355X and the same for longer rows.
-template <int C>
+template <int C, int RADIX = 8>
 struct RowCfg {
     static constexpr int NT = (512 / C) > 1 ? (512 / C) : 1;
-    static constexpr int TPB = tpb_for(NT * C, PL_ROW_RADIX);
+    static constexpr int TPB = tpb_for(NT * C, RADIX);
 };
 
 enum Kind {
@@ -1308,8 +1328,30 @@ enum Kind {
     kAdjOut = 9
 };
 
+// The steady-state outer kernels (one-launch pass, split pass) at the
+// column radix MAXR; k_adjust_out transforms nothing and has one form.
+template <int R, int NT, int MAXR>
+static void launch_outer(int kind, const Params& p, uint32_t slots, hipStream_t st)
+{
+    constexpr int TPB = SplitCfg<R, NT, MAXR>::TPB;
+    const dim3 grid(p.C / NT, slots);
+    if (kind == kInvCarry) {
+        hipLaunchKernelGGL((k_inv_carry<R, NT, TPB, MAXR>), grid, dim3(TPB), 0, st, p);
+    } else if (kind == kAdjFwd) {
+        hipLaunchKernelGGL((k_adjust_fwd<R, NT, TPB, MAXR>), grid, dim3(TPB), 0, st, p);
+    } else if (kind == kAdjOut) {
+        constexpr int TPB8 = SplitCfg<R, NT>::TPB;
+        hipLaunchKernelGGL((k_adjust_out<R, NT, TPB8>), grid, dim3(TPB8), 0, st, p);
+    } else if constexpr (NT == 8) {
+        if (p.per_slot || p.sched) hipLaunchKernelGGL((k_carry_turn<R, 8, TPB, true, MAXR>), grid, dim3(TPB), 0, st, p);
+        else hipLaunchKernelGGL((k_carry_turn<R, 8, TPB, false, MAXR>), grid, dim3(TPB), 0, st, p);
+    } else {
+        throw std::runtime_error("the one-launch outer pass needs runs of 8 digits");
+    }
+}
+
 template <int R>
-static void launch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, hipStream_t st)
+static void launch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, uint32_t radix, hipStream_t st)
 {
     if (kind == kFwdCol || kind == kInvCol) {
         constexpr int NT = ColCfg<R>::NT, TPB = ColCfg<R>::TPB;
@@ -1319,63 +1361,55 @@ static void launch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, h
         return;
     }
     if constexpr (R >= 64 && R <= 512) {
-        if (nt == 8) {
-            constexpr int TPB = SplitCfg<R, 8>::TPB;
-            dim3 grid(p.C / 8, slots);
-            if (kind == kInvCarry) hipLaunchKernelGGL((k_inv_carry<R, 8, TPB>), grid, dim3(TPB), 0, st, p);
-            else if (kind == kAdjOut) hipLaunchKernelGGL((k_adjust_out<R, 8, TPB>), grid, dim3(TPB), 0, st, p);
-            else if (kind == kCarryTurn && (p.per_slot || p.sched))
-                hipLaunchKernelGGL((k_carry_turn<R, 8, TPB, true>), grid, dim3(TPB), 0, st, p);
-            else if (kind == kCarryTurn) hipLaunchKernelGGL((k_carry_turn<R, 8, TPB, false>), grid, dim3(TPB), 0, st, p);
-            else hipLaunchKernelGGL((k_adjust_fwd<R, 8, TPB>), grid, dim3(TPB), 0, st, p);
-            return;
-        }
-        if (nt == 16) {
-            constexpr int TPB = SplitCfg<R, 16>::TPB;
-            dim3 grid(p.C / 16, slots);
-            if (kind == kInvCarry) hipLaunchKernelGGL((k_inv_carry<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
-            else if (kind == kAdjFwd) hipLaunchKernelGGL((k_adjust_fwd<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
-            else if (kind == kAdjOut) hipLaunchKernelGGL((k_adjust_out<R, 16, TPB>), grid, dim3(TPB), 0, st, p);
-            else throw std::runtime_error("the one-launch outer pass needs runs of 8 digits");
-            return;
-        }
+        if (nt == 8 && radix == 16) return launch_outer<R, 8, 16>(kind, p, slots, st);
+        if (nt == 8) return launch_outer<R, 8, 8>(kind, p, slots, st);
+        if (nt == 16 && radix == 16) return launch_outer<R, 16, 16>(kind, p, slots, st);
+        if (nt == 16) return launch_outer<R, 16, 8>(kind, p, slots, st);
     }
     throw std::runtime_error("no split outer kernels for this column length");
 }
 
 template <int C>
-static void launch_row(const Params& p, uint32_t slots, hipStream_t st)
+static void launch_row(const Params& p, uint32_t slots, uint32_t radix, hipStream_t st)
 {
-    constexpr int NT = RowCfg<C>::NT, TPB = RowCfg<C>::TPB;
-    dim3 grid(p.R / NT, slots);
-    if (p.per_slot || p.sched) hipLaunchKernelGGL((k_row<C, NT, TPB, true>), grid, dim3(TPB), 0, st, p);
-    else hipLaunchKernelGGL((k_row<C, NT, TPB, false>), grid, dim3(TPB), 0, st, p);
+    const bool multi = p.per_slot || p.sched;
+    if (radix == 16) {
+        constexpr int NT = RowCfg<C, 16>::NT, TPB = RowCfg<C, 16>::TPB;
+        const dim3 grid(p.R / NT, slots);
+        if (multi) hipLaunchKernelGGL((k_row<C, NT, TPB, true, 16>), grid, dim3(TPB), 0, st, p);
+        else hipLaunchKernelGGL((k_row<C, NT, TPB, false, 16>), grid, dim3(TPB), 0, st, p);
+    } else {
+        constexpr int NT = RowCfg<C>::NT, TPB = RowCfg<C>::TPB;
+        const dim3 grid(p.R / NT, slots);
+        if (multi) hipLaunchKernelGGL((k_row<C, NT, TPB, true, 8>), grid, dim3(TPB), 0, st, p);
+        else hipLaunchKernelGGL((k_row<C, NT, TPB, false, 8>), grid, dim3(TPB), 0, st, p);
+    }
 }
 
-static void dispatch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, hipStream_t st)
+static void dispatch_col(int kind, const Params& p, uint32_t slots, uint32_t nt, uint32_t radix, hipStream_t st)
 {
     switch (p.R) {
-    case 32: launch_col<32>(kind, p, slots, nt, st); break;
-    case 64: launch_col<64>(kind, p, slots, nt, st); break;
-    case 128: launch_col<128>(kind, p, slots, nt, st); break;
-    case 256: launch_col<256>(kind, p, slots, nt, st); break;
-    case 512: launch_col<512>(kind, p, slots, nt, st); break;
-    case 1024: launch_col<1024>(kind, p, slots, nt, st); break;
+    case 32: launch_col<32>(kind, p, slots, nt, radix, st); break;
+    case 64: launch_col<64>(kind, p, slots, nt, radix, st); break;
+    case 128: launch_col<128>(kind, p, slots, nt, radix, st); break;
+    case 256: launch_col<256>(kind, p, slots, nt, radix, st); break;
+    case 512: launch_col<512>(kind, p, slots, nt, radix, st); break;
+    case 1024: launch_col<1024>(kind, p, slots, nt, radix, st); break;
     default: throw std::runtime_error("unsupported column length");
     }
 }
 
-static void dispatch_row(const Params& p, uint32_t slots, hipStream_t st)
+static void dispatch_row(const Params& p, uint32_t slots, uint32_t radix, hipStream_t st)
 {
     switch (p.C) {
-    case 32: launch_row<32>(p, slots, st); break;
-    case 64: launch_row<64>(p, slots, st); break;
-    case 128: launch_row<128>(p, slots, st); break;
-    case 256: launch_row<256>(p, slots, st); break;
-    case 512: launch_row<512>(p, slots, st); break;
-    case 1024: launch_row<1024>(p, slots, st); break;
-    case 2048: launch_row<2048>(p, slots, st); break;
-    case 4096: launch_row<4096>(p, slots, st); break;
+    case 32: launch_row<32>(p, slots, radix, st); break;
+    case 64: launch_row<64>(p, slots, radix, st); break;
+    case 128: launch_row<128>(p, slots, radix, st); break;
+    case 256: launch_row<256>(p, slots, radix, st); break;
+    case 512: launch_row<512>(p, slots, radix, st); break;
+    case 1024: launch_row<1024>(p, slots, radix, st); break;
+    case 2048: launch_row<2048>(p, slots, radix, st); break;
+    case 4096: launch_row<4096>(p, slots, radix, st); break;
     default: throw std::runtime_error("unsupported row length");
     }
 }
@@ -1774,7 +1808,7 @@ void Group::launch(int kind, const Params& p, uint32_t slots)
 {
     launched_ = true;
     switch (kind) {
-    case kRow: dispatch_row(p, slots, stream_); break;
+    case kRow: dispatch_row(p, slots, plan_.row_radix, stream_); break;
     case kCarry:
     case kFixup: {
         const bool c = kind == kCarry, f = kind == kFixup;
@@ -1787,7 +1821,7 @@ void Group::launch(int kind, const Params& p, uint32_t slots)
     case kAdvance:
         hipLaunchKernelGGL(k_advance, dim3((slots + 63) / 64), dim3(64), 0, stream_, state_, desc_, slots);
         break;
-    default: dispatch_col(kind, p, slots, plan_.nt, stream_);
+    default: dispatch_col(kind, p, slots, plan_.nt, plan_.col_radix, stream_);
     }
 }
 

@@ -24,6 +24,7 @@
 #include "engine.h"
 #include "prewrap.h"
 #include "prothcore.h"
+#include "tune.h"
 
 namespace plpool {
 
@@ -103,29 +104,45 @@ unsigned topbit64(uint64_t k)
 // runs of 16 at the fewest bits the length admits (n just above the
 // previous length's limit).  Steps carry no multiplier (the seed is powered
 // in normal form before a slot starts stepping), so the bound takes 1.
-LenGroup* group_for(pl_pool* P, uint32_t n, uint64_t k, std::string& why)
+// The built-in plan and the map key of the group a test joins.
+bool group_plan(const pl_pool* P, uint32_t n, uint64_t k, pleng::Plan& pl, std::pair<uint32_t, uint32_t>& key,
+                std::string& why)
 {
-    pleng::Plan pl;
-    if (!pleng::choose_plan(n, k, 0, pl, why)) return nullptr;
+    if (!pleng::choose_plan(n, k, 0, pl, why)) return false;
     pleng::Plan own = pl;
-    if (!pleng::choose_layout(own, std::floor((double)n / pl.L), P->k_max, 1, true, why)) return nullptr;
+    if (!pleng::choose_layout(own, std::floor((double)n / pl.L), P->k_max, 1, true, why)) return false;
     const uint32_t nt = own.nt == 8 ? 8u : 16u;
-    const auto key = std::make_pair(pl.logL, nt);
-    auto it = P->groups.find(key);
-    if (it != P->groups.end()) return it->second.get();
+    key = std::make_pair(pl.logL, nt);
     if (nt == 8) {
         pl = own;
     } else {
         const double bmin = pl.logL > 10 ? pleng::max_bits_per_digit(pl.logL - 1, P->k_max) / 2.0 : 7.0;
-        if (!pleng::choose_layout(pl, std::max(7.0, bmin), P->k_max, 1, true, why)) return nullptr;
+        if (!pleng::choose_layout(pl, std::max(7.0, bmin), P->k_max, 1, true, why)) return false;
     }
+    return true;
+}
+
+// Slots a new group with this plan gets from what is left of the budget.
+uint32_t group_capacity(const pl_pool* P, const pleng::Plan& pl)
+{
     const size_t per_slot = pleng::Group::slot_bytes(pl);
     const uint64_t left = P->budget > P->used_bytes ? P->budget - P->used_bytes : 0;
-    uint32_t cap = (uint32_t)std::min<uint64_t>(P->max_slots, left / (per_slot + 1));
+    return (uint32_t)std::min<uint64_t>(P->max_slots, left / (per_slot + 1));
+}
+
+LenGroup* group_for(pl_pool* P, uint32_t n, uint64_t k, std::string& why)
+{
+    pleng::Plan pl;
+    std::pair<uint32_t, uint32_t> key;
+    if (!group_plan(P, n, k, pl, key, why)) return nullptr;
+    auto it = P->groups.find(key);
+    if (it != P->groups.end()) return it->second.get();
+    const uint32_t cap = group_capacity(P, pl);
     if (cap == 0) {
         why = "HBM budget exhausted";
         return nullptr;
     }
+    pltune::apply(P->device, pl, cap, true, nullptr);  // a measured plan for this key, if one was recorded
     auto lg = std::make_unique<LenGroup>();
     lg->plan = pl;
     lg->g = std::make_unique<pleng::Group>(pl, cap + 3, P->device);
@@ -428,6 +445,26 @@ int pl_pool_run(pl_pool* P, uint32_t steps, pl_pool_result* out, size_t cap, siz
     P->done.erase(P->done.begin(), P->done.begin() + (long)m);
     *n_out = m;
     return PL_OK;
+}
+
+int pl_pool_autotune(pl_pool* P, uint64_t k, uint32_t n, pl_tune_trial* trials, uint32_t max_trials,
+                     uint32_t* n_trials, pl_tune_trial* best)
+{
+    if (!P) return pfail(PL_ERR_ARG, "NULL pool");
+    try {
+        pleng::Plan pl;
+        std::pair<uint32_t, uint32_t> key;
+        std::string why;
+        if (!group_plan(P, n, k, pl, key, why)) return pfail(PL_ERR_ARG, why);
+        const uint32_t cap = group_capacity(P, pl);
+        if (cap == 0) return pfail(PL_ERR_ARG, "HBM budget exhausted");
+        pltune::Trial b;
+        const auto ts = pltune::tune(P->device, pl, cap, true, b);
+        pltune::report(ts, b, trials, max_trials, n_trials, best);
+        return PL_OK;
+    } catch (const std::exception& e) {
+        return pfail(PL_ERR_HIP, e.what());
+    }
 }
 
 int pl_pool_status(pl_pool* P, size_t* queued, size_t* active, size_t* finished_unread, uint64_t* hbm_bytes)
