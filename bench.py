@@ -81,10 +81,41 @@ def time_squarings(core, k, n, steps, warmup, verify_at=None):
     r64 = c.res64()
     kt, _ = c.kernel_times(reps=20)
     info = dict(fft_len=c.fft_len, rows=c.info.rows, cols=c.info.cols, bits_per_digit=round(c.info.bits_per_digit, 4),
-                kernels_per_squaring=c.info.passes)
+                kernels_per_squaring=c.info.passes, row_radix=c.info.row_radix, col_radix=c.info.col_radix,
+                run=c.info.run)
     c.close()
-    nt = 0 if info["kernels_per_squaring"] == 5 else 8
-    return dict(seconds=t1 - t0, max_err=emax, res64=r64, kernel_ms=kt, a=a, nt=nt, **info)
+    return dict(seconds=t1 - t0, max_err=emax, res64=r64, kernel_ms=kt, a=a, nt=info["run"], **info)
+
+
+def _plan_str(t):
+    return (f"{t['rows']}x{t['cols']} r{t['row_radix']}/c{t['col_radix']} run{t['run']} "
+            f"{'one-launch' if t['one_launch'] else 'split'}")
+
+
+def tune_single(core, k, n):
+    """The autotuner's run for a single context of (k, n) (untimed set-up,
+    as the reference's auto_tune_fft_len runs before its tests): every
+    trial's plan and device time, and the choice later contexts take."""
+    r = core.autotune_plan(k, n)
+    return dict(chosen=_plan_str(r["best"]), chosen_us=round(r["best"]["us_per_squaring"], 3),
+                trials={_plan_str(t): round(t["us_per_squaring"], 3) for t in r["trials"]},
+                all_exact=all(t["exact"] for t in r["trials"]))
+
+
+def tune_batch(core, pool, cands):
+    """Untimed set-up of a batch: the pool's group plans tuned for every
+    length group the candidates form (keys already tuned are not timed
+    again), and the single-context plan of each transform length (the plain
+    loop the batch is compared with)."""
+    groups, plain = {}, {}
+    for k, n in cands:
+        r = pool.autotune(k, n)
+        if r["trials"]:
+            groups[f"L={core.plan_fft_len(n, k)} run{r['trials'][0]['run']}"] = _plan_str(r["best"])
+        L = core.plan_fft_len(n, k)
+        if L not in plain:
+            plain[L] = tune_single(core, k, n)["chosen"]
+    return dict(pool_groups=groups, plain_contexts={str(L): v for L, v in sorted(plain.items())})
 
 
 def kernel_bytes(L: int, nt: int) -> dict:
@@ -305,6 +336,7 @@ def batch_c4(core, workloads, steps=30000):
     cands = w.candidates
     full = [_topbit(k) + n - 1 for k, n in cands]
     with core.Pool(k_max=1 << 16) as pool:
+        tuned = tune_batch(core, pool, cands)
         t0 = time.perf_counter()
         for i, (k, n) in enumerate(cands):
             pool.submit(k, n, tag=i, stop_after=steps)
@@ -325,8 +357,8 @@ def batch_c4(core, workloads, steps=30000):
                 squarings_per_s=round(thr, 1), batched_tests_per_hour=round(batched_tph, 1),
                 plain_tests_per_hour=round(plain_tph, 1), speedup_vs_plain=round(batched_tph / plain_tph, 2),
                 plain_us_per_squaring_by_fft_len={str(L): round(v * 1e6, 2) for L, v in sorted(per_len.items())},
-                max_err=max(r["max_err"] for r in res),
-                note=f"each test ran its seed powering + {steps} squarings; tests/hour extrapolates to n-1")
+                max_err=max(r["max_err"] for r in res), tuned_plans=tuned,
+                note=f"scaled: each test ran its seed powering + {steps} squarings; tests/hour extrapolates to n-1")
 
 
 def batch_c4_full(core, workloads, out_path=None):
@@ -342,6 +374,7 @@ def batch_c4_full(core, workloads, out_path=None):
     full = [_topbit(k) + n - 1 for k, n in cands]
     res = []
     with core.Pool(k_max=1 << 16) as pool:
+        tuned = tune_batch(core, pool, cands)
         t0 = time.perf_counter()
         for i, (k, n) in enumerate(cands):
             pool.submit(k, n, tag=i)
@@ -378,7 +411,7 @@ def batch_c4_full(core, workloads, out_path=None):
                           "void": sum(1 for r in res if r["error"] != 0)},
                 known_prime={"k": 3, "n": 303093, "verdict": prime_r["verdict"],
                              "res64": format(prime_r["res64"], "016X")},
-                max_err=max(r["max_err"] for r in res))
+                max_err=max(r["max_err"] for r in res), tuned_plans=tuned)
 
 
 def serve_c5(core, workloads, count=200, scale=100, load=2.0, seed=5):
@@ -393,17 +426,18 @@ def serve_c5(core, workloads, count=200, scale=100, load=2.0, seed=5):
     w = workloads.c5(count)
     cands = w.candidates
     sq = [_topbit(k) + max(1, (n - 1) // scale) for k, n in cands]
-    per_len = {}
-    for (k, n) in cands:
-        L = core.plan_fft_len(n, k)
-        if L not in per_len:
-            per_len[L] = plain_per_squaring(core, k, n, iters=768)[0]
-    svc = [per_len[core.plan_fft_len(n, k)] * s for (k, n), s in zip(cands, sq)]
-    lam = load / (sum(svc) / len(svc))
-    rng = np.random.Generator(np.random.PCG64(seed))
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(cands)))
     lat = {}
     with core.Pool(k_max=1 << 20) as pool:
+        tuned = tune_batch(core, pool, cands)
+        per_len = {}
+        for (k, n) in cands:
+            L = core.plan_fft_len(n, k)
+            if L not in per_len:
+                per_len[L] = plain_per_squaring(core, k, n, iters=768)[0]
+        svc = [per_len[core.plan_fft_len(n, k)] * s for (k, n), s in zip(cands, sq)]
+        lam = load / (sum(svc) / len(svc))
+        rng = np.random.Generator(np.random.PCG64(seed))
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(cands)))
         t0 = time.perf_counter()
         nxt = 0
         while len(lat) < len(cands):
@@ -435,7 +469,7 @@ def serve_c5(core, workloads, count=200, scale=100, load=2.0, seed=5):
                 plain_p99_s=round(float(np.percentile(L_p, 99)), 4),
                 tests_per_hour=round(3600.0 * len(cands) / makespan, 1),
                 plain_tests_per_hour=round(3600.0 * len(cands) / max(float(free_at), 1e-9), 1),
-                makespan_s=round(makespan, 3))
+                makespan_s=round(makespan, 3), tuned_plans=tuned)
 
 
 def main():
@@ -485,6 +519,8 @@ def main():
         return
 
     log(f"[bench] c2: k={k2} n={n2} steps={args.steps} warmup={args.warmup}")
+    tune2 = tune_single(core, k2, n2)
+    log(f"[bench] c2 plan: {tune2}")
     # Res64 checked after 10^4 squarings from the seed (the count the GMP
     # fixture holds) in an untimed second run, whatever --steps is
     v2 = 10000
@@ -502,6 +538,8 @@ def main():
         # Res64 checked after 10^3 squarings, the count the GMP fixture holds
         s3, v3 = 1024, 1000
         log(f"[bench] c3: k={k3} n={n3} steps={s3} warmup=256")
+        tune3 = tune_single(core, k3, n3)
+        log(f"[bench] c3 plan: {tune3}")
         r3 = time_squarings(core, k3, n3, s3, 256, verify_at=v3)
         us3 = r3["seconds"] / s3 * 1e6
         g3 = golden_res64(k3, n3, v3)
@@ -514,7 +552,7 @@ def main():
                               kernel_ms=r3["kernel_ms"], dominant_kernel=dom3, roofline=roof3,
                               kernels_per_squaring=r3["kernels_per_squaring"], bytes_per_squaring=sq3_bytes,
                               hbm_roofline_us=round(roofline_us, 3), x_of_roofline=round(us3 / roofline_us, 2),
-                              fp64=fp64_rate(r3["fft_len"], us3))
+                              fp64=fp64_rate(r3["fft_len"], us3), plan_tuning=tune3)
         log(f"[bench] c3: {us3:.2f} us/squaring, max_err={r3['max_err']:.4g}")
 
     if not args.no_batch:
@@ -542,7 +580,9 @@ def main():
         "config": {"workload": "c2_n2p21 single test: n=2^21, k odd in [3,2^20), 1 squaring per step",
                    "k": k2, "n": n2, "base": r2["a"], "fft_len": r2["fft_len"], "rows": r2["rows"],
                    "cols": r2["cols"], "bits_per_digit": r2["bits_per_digit"],
-                   "kernels_per_squaring": r2["kernels_per_squaring"]},
+                   "kernels_per_squaring": r2["kernels_per_squaring"], "row_radix": r2["row_radix"],
+                   "col_radix": r2["col_radix"], "carry_run": r2["run"]},
+        "plan_tuning": tune2,
         "res64_after": v2,
         "res64": format(r2["res64"], "016X"),
         "res64_matches_gmp": None if g2 is None else r2["res64"] == g2,

@@ -360,7 +360,9 @@ int pl_pool_run(pl_pool* pool, uint32_t steps, pl_pool_result* out, size_t cap, 
 int pl_pool_status(pl_pool* pool, size_t* queued, size_t* active, size_t* finished_unread, uint64_t* hbm_bytes);
 /* Tunes the plan of the length group a candidate (k, n) would join, with
  * the slot count that group would get from this pool's budget (see
- * pl_autotune_plan).  A group that already exists keeps its plan. */
+ * pl_autotune_plan).  A key the table already holds is not timed again
+ * (*n_trials = 0, *best = the recorded choice), so it can be called for
+ * every candidate of a batch.  A group that already exists keeps its plan. */
 int pl_pool_autotune(pl_pool* pool, uint64_t k, uint32_t n, pl_tune_trial* trials, uint32_t max_trials,
                      uint32_t* n_trials, pl_tune_trial* best);
 const char* pl_pool_last_error(void);

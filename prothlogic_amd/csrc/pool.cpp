@@ -459,6 +459,18 @@ int pl_pool_autotune(pl_pool* P, uint64_t k, uint32_t n, pl_tune_trial* trials, 
         const uint32_t cap = group_capacity(P, pl);
         if (cap == 0) return pfail(PL_ERR_ARG, "HBM budget exhausted");
         pltune::Trial b;
+        pltune::Choice have;
+        if (pltune::get(pltune::key_for(P->device, pl, cap, true), have)) {
+            // tuned already (by this pool or another in the process): report it
+            b.c = have;
+            pleng::Plan tp = pl;
+            pltune::apply_choice(tp, have);
+            b.R = tp.R;
+            b.C = tp.C;
+            b.exact = true;
+            pltune::report({}, b, trials, max_trials, n_trials, best);
+            return PL_OK;
+        }
         const auto ts = pltune::tune(P->device, pl, cap, true, b);
         pltune::report(ts, b, trials, max_trials, n_trials, best);
         return PL_OK;
