@@ -4,18 +4,24 @@ one MI355X, with the residue resident in HBM.
 Workload (default): the n = 2^21 single large test -- k drawn odd in
 [3, 2^20) from PCG64(seed=2) (gate survivors only), base a by the reference's
 rule, seed a^k mod N built on the device.  A "step" is one squaring
-x <- x^2 mod N.  W untimed warm-up squarings run on the same context from
-the seed (the default, 256, is one whole chunk, so the engine has captured
-its chunk graph before the clock starts); then exactly the next K squarings
-are timed with the device synchronised on both sides.  Res64 after 10^4
-squarings from the seed (an untimed second run) is checked against the GMP
-fixture.
+x <- x^2 mod N.  Before anything is timed the launch-plan autotuner picks
+the plan (shape, radices, outer pass) for the workload's length.  W untimed
+warm-up squarings run on the same context from the seed (the default, 512,
+is two 256-squaring chunks: the first runs partly in normal form while the
+seed's exact value is small, the second is whole and captures the chunk
+graph before the clock starts); then exactly the next K squarings are timed
+with the device synchronised on both sides.  Res64 after 10^4 squarings from
+the seed (an untimed second run) is checked against the GMP fixture.
 
 The same line reports the n = 2^24 test (1024 squarings timed, Res64 checked
-after 10^3), the per-kernel roofline (HIP events on the engine's stream), and
-the reference CPU engine timed on this host for a bounded sample.
+after 10^3), the per-kernel roofline (HIP events on the engine's stream), the
+batched (config 4, scaled) and serving (config 5, scaled) legs, and the CPU
+legs (the reference's own sources and the C restatement, timed on this
+host's cores for bounded samples).
 
-  python bench.py [--gpus 1] [--steps 10000] [--warmup 256] [--no-cpu]
+  python bench.py [--gpus 1] [--steps 10000] [--warmup 512] [--no-cpu] [--no-tune]
+  python bench.py --c4-full OUT.json     config 4 at its stated size
+  python bench.py --cpu-only             the CPU legs alone
 """
 from __future__ import annotations
 
@@ -92,10 +98,15 @@ def _plan_str(t):
             f"{'one-launch' if t['one_launch'] else 'split'}")
 
 
+TUNE = True  # bench.py --no-tune clears it
+
+
 def tune_single(core, k, n):
     """The autotuner's run for a single context of (k, n) (untimed set-up,
     as the reference's auto_tune_fft_len runs before its tests): every
     trial's plan and device time, and the choice later contexts take."""
+    if not TUNE:
+        return dict(chosen="not tuned (--no-tune)")
     r = core.autotune_plan(k, n)
     return dict(chosen=_plan_str(r["best"]), chosen_us=round(r["best"]["us_per_squaring"], 3),
                 trials={_plan_str(t): round(t["us_per_squaring"], 3) for t in r["trials"]},
@@ -108,6 +119,8 @@ def tune_batch(core, pool, cands):
     again), and the single-context plan of each transform length (the plain
     loop the batch is compared with)."""
     groups, plain = {}, {}
+    if not TUNE:
+        return dict(note="not tuned (--no-tune)")
     for k, n in cands:
         r = pool.autotune(k, n)
         if r["trials"]:
@@ -307,11 +320,13 @@ def _topbit(k: int) -> int:
 def plain_per_squaring(core, k, n, iters=1536):
     """Steady-state seconds per squaring of one candidate alone on the GPU
     (the plain one-candidate-at-a-time loop: one context, graph-captured).
-    One untimed chunk captures the chunk graph (a one-time cost that a full
-    test of 10^5..10^6 squarings amortises); whole chunks are timed."""
+    Two untimed chunks (the first runs partly in normal form while a^3 is
+    small, as direct launches; the second is whole and captures the chunk
+    graph, a one-time cost that a full test of 10^5..10^6 squarings
+    amortises); whole chunks are timed."""
     c = core.Context(k, n)
     c.set_residue(core.choose_base(k, n) ** 3)
-    c.square(256)
+    c.square(512)
     t0 = time.perf_counter()
     c.square(iters)
     dt = time.perf_counter() - t0
@@ -476,7 +491,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=256)
+    ap.add_argument("--warmup", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the n=2^24 line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (config 4) leg")
@@ -484,6 +499,11 @@ def main():
     ap.add_argument("--c4-full", metavar="OUT.json", default=None,
                     help="only run config 4 at its stated size (every test to its verdict); per-test results to OUT")
     ap.add_argument("--cpu-only", action="store_true", help="only the CPU legs (no GPU needed)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the launch-plan autotuner (plans from --plans, else the built-in rule); for "
+                         "profiler runs, whose per-dispatch tracing makes the tuner's timings meaningless")
+    ap.add_argument("--plans", metavar="FILE", default=None, help="load a tuned-plan table first")
+    ap.add_argument("--save-plans", metavar="FILE", default=None, help="write the tuned-plan table at the end")
     ap.add_argument("--cpu-sample", nargs=4, metavar=("KIND", "K", "N", "SECONDS"), help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_sample:
@@ -514,6 +534,10 @@ def main():
     if core.device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
+    global TUNE
+    TUNE = not args.no_tune
+    if args.plans:
+        core.tune_load(args.plans)
     if args.c4_full:
         print(json.dumps(batch_c4_full(core, workloads, args.c4_full)), flush=True)
         return
@@ -536,11 +560,14 @@ def main():
         (k3, n3), = workloads.c3().candidates
         # four whole 256-squaring chunks timed after one untimed chunk;
         # Res64 checked after 10^3 squarings, the count the GMP fixture holds
-        s3, v3 = 1024, 1000
-        log(f"[bench] c3: k={k3} n={n3} steps={s3} warmup=256")
+        # 512 warm-up squarings: the first chunk is partly in normal form (the
+        # seed's exact value is small) and runs as direct launches; the second
+        # is a whole one, which captures the chunk graph before the clock starts
+        s3, v3, w3 = 1024, 1000, 512
+        log(f"[bench] c3: k={k3} n={n3} steps={s3} warmup={w3}")
         tune3 = tune_single(core, k3, n3)
         log(f"[bench] c3 plan: {tune3}")
-        r3 = time_squarings(core, k3, n3, s3, 256, verify_at=v3)
+        r3 = time_squarings(core, k3, n3, s3, w3, verify_at=v3)
         us3 = r3["seconds"] / s3 * 1e6
         g3 = golden_res64(k3, n3, v3)
         dom3, roof3, dom3_bytes, sq3_bytes = roofline(r3["kernel_ms"], r3["fft_len"], r3["nt"])
@@ -598,6 +625,8 @@ def main():
         **extra,
     }
     print(json.dumps(line), flush=True)
+    if args.save_plans:
+        core.tune_save(args.save_plans)
 
 
 if __name__ == "__main__":

@@ -124,7 +124,9 @@ int pl_tune_load(const char* path);
  * The residue starts at 0.  Environment (read at creation): PL_CARRY_TURN=0
  * runs the outer pass as two launches (3 kernels per squaring instead of 2);
  * PL_FUSED=0 runs it unfused (5 kernels).  pl_info.passes reports which
- * form the context prefers. */
+ * form the context prefers.  PL_NO_GRAPHS=1 (contexts and pool groups)
+ * launches whole 256-squaring chunks directly instead of replaying their
+ * captured graphs: the same kernels, for dispatch-tracing profilers. */
 int pl_ctx_create(uint64_t k, uint32_t n, uint32_t fft_len, int device, pl_ctx** out);
 void pl_ctx_destroy(pl_ctx* ctx);
 int pl_ctx_info(const pl_ctx* ctx, pl_info* out);

@@ -223,6 +223,10 @@ class Group {
     bool uniform_ = true;  // every slot holds the plan's n and k
     bool turn_ = true;     // split outer pass as one launch (k_carry_turn); PL_CARRY_TURN=0: two
     bool give_up_ = false; // PL_HANDOFF_GIVE_UP=1: every carry hand-off wait gives up (tests)
+    // PL_NO_GRAPHS=1: whole chunks run as direct launches like partial ones
+    // (the same kernels in the same order; for tools that trace dispatches
+    // and must not see graph replays)
+    bool no_graphs_ = false;
     uint32_t fallbacks_ = 0;
     bool owns_ = true;     // the memory and the stream are the group's own
     // the status word as read at the last wait on the stream, and whether

@@ -1656,6 +1656,7 @@ void Group::init(void* ext_mem, size_t ext_bytes, hipStream_t ext_stream)
         if (const char* e = std::getenv("PL_CARRY_TURN")) turn_ = e[0] != '0';
     }
     if (const char* e = std::getenv("PL_HANDOFF_GIVE_UP")) give_up_ = e[0] == '1';
+    if (const char* e = std::getenv("PL_NO_GRAPHS")) no_graphs_ = e[0] == '1';
     eb_ = lay.eb;
     const size_t o_z = lay.z, o_car = lay.car, o_runc = lay.runc, o_runt = lay.runt, o_epoch = lay.epoch;
     const size_t o_rdig = lay.rdig, o_xhi = lay.xhi, o_err = lay.err, o_errp = lay.errp, o_st = lay.st;
@@ -2029,7 +2030,7 @@ double Group::square(uint64_t iters, double* err_trace) { return square_slots(0,
 // steady squarings, exit -- measured slower on MI355X than the launches.)
 void Group::run_plain_chunk(int g, bool use_turn)
 {
-    if (g == kChunk) {
+    if (g == kChunk && !no_graphs_) {
         const uint64_t key = ((uint64_t)sq0_ << 33) | ((uint64_t)sqn_ << 1) | (use_turn ? 1u : 0u);
         auto it = plain_graphs_.find(key);
         if (it == plain_graphs_.end()) {
@@ -2168,7 +2169,7 @@ void Group::step_begin(uint32_t g, uint32_t slots)
         ns = std::min(ns, cap_);
     }
     step_n_ = ns;
-    if (g == (uint32_t)kStepChunk) {
+    if (g == (uint32_t)kStepChunk && !no_graphs_) {
         auto it = step_graphs_.find(ns);
         if (it == step_graphs_.end()) {
             hipGraphExec_t ex = nullptr;
