@@ -49,9 +49,12 @@ typedef struct pl_info {
     uint32_t cols;         /* C: contiguous DFT length                      */
     double bits_per_digit; /* n / L                                         */
     uint64_t device_bytes; /* HBM held by the context                       */
-    uint32_t passes;       /* kernels per steady-state squaring: 2 with the
-                              one-launch outer pass, 3 with it in two
-                              launches (PL_CARRY_TURN=0), 5 unfused          */
+    uint32_t passes;       /* kernels per steady-state squaring: 0 when the
+                              steady state runs as one resident launch per
+                              chunk of squarings (PL_RESIDENT=0 turns it
+                              off), 2 with the one-launch outer pass, 3 with
+                              it in two launches (PL_CARRY_TURN=0), 5
+                              unfused                                        */
     uint32_t handoff_fallbacks; /* chunks recomputed on the two-launch pass
                               after a carry hand-off wait gave up            */
     uint32_t row_radix;    /* largest radix of the steady-state row pass    */

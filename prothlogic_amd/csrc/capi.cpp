@@ -226,6 +226,10 @@ int create_ctx(uint64_t k, uint32_t n, uint32_t fft_len, int device, uint32_t ca
         if (!rule_plan(n, k, fft_len, c->plan, why)) return fail(PL_ERR_ARG, why);
         bool turn = true;
         const bool tuned = pltune::apply(device, c->plan, 1, false, &turn);
+        if (!tuned && c->plan.nt == 8) {
+            // untuned: the shape whose steady state runs as one resident launch
+            if (const uint32_t lr = pleng::resident_logR(device, c->plan.L)) pleng::set_shape(c->plan, lr);
+        }
         c->eng = std::make_unique<pleng::Group>(c->plan, capacity + 3, device);
         if (tuned && !turn) c->eng->no_turn();
         c->scr = plprep::Scratch{ capacity, capacity + 1, capacity + 2 };

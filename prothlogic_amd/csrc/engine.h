@@ -37,6 +37,9 @@ struct Plan {
 std::vector<uint32_t> shape_options(uint32_t logL, bool split);
 // Sets R = 2^logR, C = L/R (and the two-level table split) on a plan.
 void set_shape(Plan& p, uint32_t logR);
+// log2 R of the preferred shape whose steady state runs as one resident
+// launch (k_loop) at length L on this device; 0 when no such shape fits.
+uint32_t resident_logR(int device, uint32_t L);
 
 // Smallest power of two L in [2^10, 2^22] with n/L <= max_bits_per_digit(L, k)
 // and n/L >= 7.  force_L, when nonzero, is checked and used instead.
@@ -131,10 +134,30 @@ class Group {
     void clear_status();
     // kernels per steady-state squaring: 2 (one-launch outer pass + row
     // pass), 3 (split outer pass in two launches + row pass) or 5 (unfused)
-    int launches_per_squaring() const { return turn() ? 2 : (plan_.nt ? 3 : 5); }
+    // (0: the steady state is one resident launch per chunk, k_loop)
+    int launches_per_squaring() const
+    {
+        return resident_ && turn() && uniform_ && plan_.row_radix == 8 && plan_.col_radix == 8 ? 0
+             : turn()                                                                       ? 2
+             : plan_.nt                                                                     ? 3
+                                                                                            : 5;
+    }
     // the one-launch outer pass can be used (runs of 8 digits; with 16 its
     // registers would spill)
     bool turn() const { return turn_ && plan_.nt == 8; }
+    // Plain chunks of this group's slot run their steady state as one
+    // resident launch (k_loop: every workgroup on its CU for the whole chunk,
+    // outer and row passes handing points over without launch boundaries).
+    // Needs the one-launch pass, a shape k_loop is built for whose workgroups
+    // all fit on the device at once, one slot per chunk and every slot
+    // holding the plan's candidate, and the plan's radices at 8 (the loop
+    // variants fix their own; plans pinned at radix 16 keep two kernels).
+    // PL_RESIDENT=0 turns it off.
+    bool resident_on() const
+    {
+        return resident_ && turn() && sqn_ == 1 && uniform_ && plan_.row_radix == 8 && plan_.col_radix == 8;
+    }
+    void no_resident() { resident_ = false; }
     // Use the split outer pass even where the one-launch pass would run.  The
     // one-launch pass's block 0 waits on its slot's last block, which starts
     // after it: a launch only finishes when all its blocks can eventually be
@@ -202,6 +225,7 @@ class Group {
     unsigned* epoch_ = nullptr;   // cap x 2: next and last carry hand-off epoch
     int* rdig_ = nullptr;
     long long* xhi_ = nullptr;
+    unsigned long long* xw_ = nullptr;  // per slot: X_hi as a tagged word (k_loop's row passes poll it)
     unsigned long long* err_ = nullptr;   // cap x kChunk (bit patterns of non-negative doubles)
     unsigned long long* errp_ = nullptr;  // cap x eb_ x kChunk per-block partials
     uint32_t eb_ = 1;                     // blocks that report round-off per squaring
@@ -222,6 +246,9 @@ class Group {
     uint32_t pending_steps_ = 0;  // steps enqueued by step_begin, not yet read back
     bool uniform_ = true;  // every slot holds the plan's n and k
     bool turn_ = true;     // split outer pass as one launch (k_carry_turn); PL_CARRY_TURN=0: two
+    bool resident_ = false;  // k_loop exists for the shape and fits the device (resident_on())
+    int loop_var_ = -1;      // its variant (engine.hip kLoopShapes)
+    unsigned poll_sleep_ = 1;  // k_loop's pause between reloads of points not yet ready
     bool give_up_ = false; // PL_HANDOFF_GIVE_UP=1: every carry hand-off wait gives up (tests)
     // PL_NO_GRAPHS=1: whole chunks run as direct launches like partial ones
     // (the same kernels in the same order; for tools that trace dispatches

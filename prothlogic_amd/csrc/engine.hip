@@ -107,6 +107,10 @@ struct Params {
     int per_slot;           // slots hold different candidates: read n, k from desc
                             // (else n, k below are every slot's, and no kernel
                             // starts with a dependent descriptor load)
+    int ztag;               // row pass: store points with ready parity 1 (the pass
+                            // entering a k_loop launch)
+    unsigned poll_sleep;    // k_loop: pause between reloads of points not yet ready
+                            // (units of s_sleep 1, about 64 clocks)
     // bound per slot by bind()
     unsigned long long n;
     long long k;
@@ -161,6 +165,14 @@ __device__ __forceinline__ Params bind(const Params& base, unsigned y)
 }
 
 // ------------------------------------------------------------- helpers
+
+// v with the lowest mantissa bit of both parts set: the ready parity of a
+// row pass's points (resident.h).
+__device__ __forceinline__ double2 with_parity1(double2 v)
+{
+    return make_double2(__longlong_as_double(__double_as_longlong(v.x) | 1ll),
+                        __longlong_as_double(__double_as_longlong(v.y) | 1ll));
+}
 
 __device__ __forceinline__ double2 twL(const Params& p, unsigned j)  // W_L^j, j < L
 {
@@ -371,12 +383,13 @@ struct ColTw {
     // column c (k_carry_turn folds the column weight in here: the column DFT
     // is linear, so scaling its input or output by f scales the whole column)
     template <bool SCALED = false>
-    __device__ __forceinline__ void load(const Params& p, unsigned c, double2 f = make_double2(1.0, 0.0))
+    __device__ __forceinline__ void load(const Params& p, unsigned c, double2 f = make_double2(1.0, 0.0),
+                                         unsigned tid = threadIdx.x)
     {
 #pragma unroll
         for (int u = 0; u < S::PER; ++u) {
-            if (!S::live(u)) continue;
-            double2 a = twL(p, c * (unsigned)S::j(u));
+            if (!S::live(u, tid)) continue;
+            double2 a = twL(p, c * (unsigned)S::j(u, tid));
             if constexpr (SCALED) a = cmul(a, f);
             const double2 q = p.tcq[(size_t)c * 16 + 16 / P];
             w[u][0] = a;
@@ -457,7 +470,9 @@ __global__ __launch_bounds__(TPB) void k_row(Params pb)
     dft_square_pass_pp<C, NT, TPB, RADIX>(
         buf, t, [&](int tr, int idx, int, int) { return z[base + (size_t)tr * C + idx]; },
         [&](double2 X) { return cmul(X, make_double2(X.x, X.y + x2)); },
-        [&](int tr, int idx, double2 v, int, int) { store_stream(&p.z[base + (size_t)tr * C + idx], v); });
+        [&](int tr, int idx, double2 v, int, int) {
+            store_stream(&p.z[base + (size_t)tr * C + idx], p.ztag ? with_parity1(v) : v);
+        });
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         p.epoch[0] = last + 1u;
         p.epoch[2] = col + 1u;  // the next outer pass carries the next squaring
@@ -1065,6 +1080,10 @@ __global__ __launch_bounds__(TPB) void k_carry_turn(Params pb)
     else dft<R, NT, TPB, false, true, false>(io, t, ld, st);
 }
 
+}  // namespace pleng
+#include "resident.h"
+namespace pleng {
+
 __global__ void k_set_digits(Params pb, const int* __restrict__ d)
 {
     const Params p = bind(pb, blockIdx.y);
@@ -1436,6 +1455,78 @@ static void dispatch_row_mul(const Params& p, const double2* zo, const long long
     }
 }
 
+// Resident-loop variants: shape R x C, tile widths, threads, column and row
+// radix.  One column tile of NTD columns and one row tile of NTR rows per
+// workgroup (C/NTD = R/NTR workgroups); the carry phase takes one run of 8
+// digits per thread (threads past the tile's runs wait).
+struct LoopShape {
+    uint32_t R, C, ntd, ntr, tpb, cr, rr;
+    const void* kern;
+};
+#define PL_LOOP(R_, C_, NTD_, NTR_, T_, CR_, RR_) \
+    { R_, C_, NTD_, NTR_, T_, CR_, RR_, reinterpret_cast<const void*>(&k_loop<R_, C_, NTD_, NTR_, T_, CR_, RR_>) }
+static const LoopShape kLoopShapes[] = {
+    // preferred variant of each shape first (measured on MI355X)
+    PL_LOOP(64, 512, 8, 1, 128, 4, 4),      // L = 2^15: 64 workgroups
+    PL_LOOP(128, 512, 8, 2, 256, 4, 4),     // L = 2^16: 64 workgroups
+    PL_LOOP(128, 1024, 8, 1, 256, 4, 4),    // L = 2^17: 128 workgroups
+    PL_LOOP(128, 1024, 8, 1, 128, 8, 8),
+    PL_LOOP(256, 512, 8, 4, 256, 8, 8),     // L = 2^17: 64 workgroups
+    PL_LOOP(128, 2048, 16, 1, 512, 4, 4),   // L = 2^18: 128 workgroups
+    PL_LOOP(256, 1024, 8, 2, 256, 8, 8),    // L = 2^18: 128 workgroups
+    PL_LOOP(256, 2048, 16, 2, 512, 8, 8),   // L = 2^19: 128 workgroups
+    PL_LOOP(256, 4096, 16, 1, 512, 8, 8),   // L = 2^20: 256 workgroups
+};
+#undef PL_LOOP
+constexpr int kNumLoopShapes = (int)(sizeof(kLoopShapes) / sizeof(kLoopShapes[0]));
+
+// Whether variant i's workgroups all fit on the device at once.
+static bool loop_fits(int device, int i)
+{
+    const LoopShape& sh = kLoopShapes[i];
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sh.kern, (int)sh.tpb, 0) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return false;
+    return (long long)per_cu * cus >= (long long)(sh.C / sh.ntd);
+}
+
+// The first variant for this shape that fits (PL_LOOP_VAR=i picks variant i
+// when it has the shape); -1 when none does.
+static int loop_variant(int device, uint32_t R, uint32_t C)
+{
+    if (const char* e = std::getenv("PL_LOOP_VAR")) {
+        const int i = std::atoi(e);
+        if (i >= 0 && i < kNumLoopShapes && kLoopShapes[i].R == R && kLoopShapes[i].C == C && loop_fits(device, i))
+            return i;
+    }
+    for (int i = 0; i < kNumLoopShapes; ++i)
+        if (kLoopShapes[i].R == R && kLoopShapes[i].C == C && loop_fits(device, i)) return i;
+    return -1;
+}
+
+uint32_t resident_logR(int device, uint32_t L)
+{
+    if (const char* e = std::getenv("PL_RESIDENT"))
+        if (e[0] == '0') return 0;
+    check(hipSetDevice(device), "hipSetDevice");
+    for (int i = 0; i < kNumLoopShapes; ++i)
+        if (kLoopShapes[i].R * kLoopShapes[i].C == L && loop_fits(device, i)) {
+            uint32_t l = 0;
+            while ((1u << l) < kLoopShapes[i].R) ++l;
+            return l;
+        }
+    return 0;
+}
+
+static void launch_loop(int var, const Params& p, unsigned nsq, unsigned long long* xw, hipStream_t st)
+{
+    if (var < 0 || var >= kNumLoopShapes) throw std::runtime_error("no resident loop for this shape");
+    const LoopShape& sh = kLoopShapes[var];
+    Params pa = p;
+    void* args[] = { &pa, &nsq, &xw };
+    check(hipLaunchKernel(sh.kern, dim3(sh.C / sh.ntd), dim3(sh.tpb), args, 0, st), "k_loop launch");
+}
+
 template <int RUN, int TPB>
 static void launch_carry_fixup(const Params& p, unsigned blocks, uint32_t slots, bool carry, bool fixup, hipStream_t st)
 {
@@ -1564,7 +1655,7 @@ size_t Group::slot_bytes(const Plan& p)
 
 // Byte offsets of a group's buffers in its one device allocation.
 struct GroupLayout {
-    size_t z, car, runc, runt, epoch, rdig, xhi, err, errp, st, dig, desc, state, zero_end;
+    size_t z, car, runc, runt, epoch, rdig, xhi, xw, err, errp, st, dig, desc, state, zero_end;
     size_t tw, tlo, thi, olo, ohi, plo, phi, ilo, ihi, tcq, wrf, wri, wcf, wci, bytes;
     uint32_t carry_tpb, carry_blocks, eb;
 };
@@ -1590,6 +1681,7 @@ static GroupLayout group_layout(const Plan& p, uint32_t cap)
     put(g.epoch, sizeof(unsigned) * 4 * cap);
     put(g.rdig, p.nt ? sizeof(int) * (size_t)p.L * cap : 0);
     put(g.xhi, sizeof(long long) * cap);
+    put(g.xw, sizeof(unsigned long long) * cap);
     put(g.err, sizeof(unsigned long long) * Group::kChunk * cap);
     put(g.errp, sizeof(unsigned long long) * Group::kChunk * g.eb * cap);
     put(g.st, sizeof(int));
@@ -1657,6 +1749,10 @@ void Group::init(void* ext_mem, size_t ext_bytes, hipStream_t ext_stream)
     }
     if (const char* e = std::getenv("PL_HANDOFF_GIVE_UP")) give_up_ = e[0] == '1';
     if (const char* e = std::getenv("PL_NO_GRAPHS")) no_graphs_ = e[0] == '1';
+    loop_var_ = owns_ ? loop_variant(device_, p.R, p.C) : -1;
+    resident_ = loop_var_ >= 0;
+    if (const char* e = std::getenv("PL_POLL_SLEEP")) poll_sleep_ = (unsigned)std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("PL_RESIDENT")) resident_ = resident_ && e[0] != '0';
     eb_ = lay.eb;
     const size_t o_z = lay.z, o_car = lay.car, o_runc = lay.runc, o_runt = lay.runt, o_epoch = lay.epoch;
     const size_t o_rdig = lay.rdig, o_xhi = lay.xhi, o_err = lay.err, o_errp = lay.errp, o_st = lay.st;
@@ -1676,6 +1772,7 @@ void Group::init(void* ext_mem, size_t ext_bytes, hipStream_t ext_stream)
     epoch_ = reinterpret_cast<unsigned*>(b + o_epoch);
     rdig_ = reinterpret_cast<int*>(b + o_rdig);
     xhi_ = reinterpret_cast<long long*>(b + o_xhi);
+    xw_ = reinterpret_cast<unsigned long long*>(b + lay.xw);
     err_ = reinterpret_cast<unsigned long long*>(b + o_err);
     errp_ = reinterpret_cast<unsigned long long*>(b + o_errp);
     status_ = reinterpret_cast<int*>(b + o_st);
@@ -1791,6 +1888,8 @@ Params Group::params(uint32_t slot0, bool check_active) const
     p.spin_limit = give_up_ ? 64u : kSpinLimit;
     p.tag_skew = give_up_ ? 1u : 0u;
     p.per_slot = uniform_ ? 0 : 1;
+    p.ztag = 0;
+    p.poll_sleep = poll_sleep_;
     p.n = plan_.n;
     p.k = (long long)plan_.k;
     p.mult = 1;
@@ -1909,6 +2008,19 @@ void Group::enqueue_plain_chunk(int g, bool use_turn)
                            xhi_ + s0, xbak_, ns);
     }
     launch(kFwdCol, params(s0, false), ns);
+    if (use_turn && g >= 2 && resident_on()) {
+        // the steady state as one resident launch: the entry row pass stores
+        // ready-tagged points, k_loop runs the g-1 steady squarings
+        Params pe = params(s0, false);
+        pe.ztag = 1;
+        launch(kRow, pe, ns);
+        Params pt = params(s0, false);
+        pt.runc = runt_;
+        launched_ = true;
+        launch_loop(loop_var_, pt, (unsigned)(g - 1), xw_ + s0, stream_);
+        enqueue_exit(params(s0, false), ns);
+        return;
+    }
     launch(kRow, params(s0, false), ns);
     for (int i = 0; i + 1 < g; ++i) {
         const Params p = params(s0, false);

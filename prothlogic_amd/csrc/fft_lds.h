@@ -220,8 +220,11 @@ struct StageShape {
     static constexpr int PER = NBT >= G ? NBT / G : 1;
     static_assert(TPB % NT == 0, "TPB must be a multiple of NT");
     static_assert(NBT < G || NBT % G == 0, "butterflies must split evenly");
-    __device__ static __forceinline__ int j(int u) { return (int)(threadIdx.x / NT) + u * G; }
-    __device__ static __forceinline__ bool live(int u) { return NBT >= G || j(u) < NBT; }
+    // tid: the thread's index in the workgroup (a caller that loops over
+    // whole transforms passes an opaque copy, so that the address arithmetic
+    // derived from it is not hoisted out of the loop and kept live across it)
+    __device__ static __forceinline__ int j(int u, unsigned tid = threadIdx.x) { return (int)(tid / NT) + u * G; }
+    __device__ static __forceinline__ bool live(int u, unsigned tid = threadIdx.x) { return NBT >= G || j(u, tid) < NBT; }
 };
 
 // Register-resident stage twiddles of one thread for every stage of an
@@ -258,7 +261,7 @@ __device__ __forceinline__ void twiddles(const double2 (&b)[4], double2* w)
 }
 
 template <int N, int NT, int TPB, int MAXR, int NS, int SI>
-__device__ __forceinline__ void tw_fill(StageTw<N, NT, TPB, MAXR>& t, const double2* __restrict__ tw)
+__device__ __forceinline__ void tw_fill(StageTw<N, NT, TPB, MAXR>& t, const double2* __restrict__ tw, unsigned tid)
 {
     constexpr int P = Radix<N, MAXR>::of(NS);
     if constexpr (NS > 1) {
@@ -267,41 +270,42 @@ __device__ __forceinline__ void tw_fill(StageTw<N, NT, TPB, MAXR>& t, const doub
         constexpr int STEP = N / (NS * P);
 #pragma unroll
         for (int u = 0; u < S::PER; ++u) {
-            if (!S::live(u)) continue;
+            if (!S::live(u, tid)) continue;
             // e < (N/P) * SCALE = 4096/P: the index stays in the table.
             // W^e from the table, W^2e, W^4e, W^8e by squaring: one load per
             // stage and butterfly instead of up to four (each load a thread
             // issues before its data costs ~0.2 us at L = 2^20; the squarings
             // add a few ulp to a twiddle, far inside the round-off guard)
-            const int e = (S::j(u) % NS) * STEP * SCALE;
+            const int e = (S::j(u, tid) % NS) * STEP * SCALE;
             t.w[SI][u][0] = tw[e];
             if constexpr (P >= 4) t.w[SI][u][1] = cmul(t.w[SI][u][0], t.w[SI][u][0]);
             if constexpr (P >= 8) t.w[SI][u][2] = cmul(t.w[SI][u][1], t.w[SI][u][1]);
             if constexpr (P >= 16) t.w[SI][u][3] = cmul(t.w[SI][u][2], t.w[SI][u][2]);
         }
     }
-    if constexpr (NS * P < N) tw_fill<N, NT, TPB, MAXR, NS * P, SI + 1>(t, tw);
+    if constexpr (NS * P < N) tw_fill<N, NT, TPB, MAXR, NS * P, SI + 1>(t, tw, tid);
 }
 
 template <int N, int NT, int TPB, int MAXR>
-__device__ __forceinline__ void load_stage_tw(StageTw<N, NT, TPB, MAXR>& t, const double2* __restrict__ tw)
+__device__ __forceinline__ void load_stage_tw(StageTw<N, NT, TPB, MAXR>& t, const double2* __restrict__ tw,
+                                              unsigned tid = threadIdx.x)
 {
-    tw_fill<N, NT, TPB, MAXR, 1, 0>(t, tw);
+    tw_fill<N, NT, TPB, MAXR, 1, 0>(t, tw, tid);
 }
 
 // One stage.  ld(tr, idx, u, r) -> double2, st(tr, idx, v, u, r).  When
 // both go to the same LDS buffer the caller passes SYNC_BETWEEN = true so
 // every read of the stage completes before any write.
 template <int N, int NT, int TPB, int MAXR, int P, int NS, int SI, bool INV, bool SYNC_BETWEEN, class Ld, class St>
-__device__ __forceinline__ void stage(const StageTw<N, NT, TPB, MAXR>& t, Ld ld, St st)
+__device__ __forceinline__ void stage(const StageTw<N, NT, TPB, MAXR>& t, Ld ld, St st, unsigned tid)
 {
     using S = StageShape<N, NT, TPB, P>;
-    const int tr = threadIdx.x % NT;
+    const int tr = tid % NT;
     double2 v[S::PER][P];
 #pragma unroll
     for (int u = 0; u < S::PER; ++u) {
-        if (S::live(u)) {
-            const int j = S::j(u);
+        if (S::live(u, tid)) {
+            const int j = S::j(u, tid);
 #pragma unroll
             for (int r = 0; r < P; ++r) v[u][r] = ld(tr, j + r * S::NBT, u, r);
             if constexpr (NS > 1) {
@@ -316,8 +320,8 @@ __device__ __forceinline__ void stage(const StageTw<N, NT, TPB, MAXR>& t, Ld ld,
     if constexpr (SYNC_BETWEEN) __syncthreads();
 #pragma unroll
     for (int u = 0; u < S::PER; ++u) {
-        if (S::live(u)) {
-            const int j = S::j(u);
+        if (S::live(u, tid)) {
+            const int j = S::j(u, tid);
             const int o = (j / NS) * NS * P + (j % NS);
 #pragma unroll
             for (int r = 0; r < P; ++r) st(tr, o + r * NS, v[u][r], u, r);
@@ -332,28 +336,29 @@ __device__ __forceinline__ void stage(const StageTw<N, NT, TPB, MAXR>& t, Ld ld,
 // stage's reads and writes).  The caller provides the barrier before the
 // call when LDS was in use; the call ends without a trailing barrier.
 template <int N, int NT, int TPB, int MAXR, bool INV, bool IN_LDS, bool OUT_LDS, int NS, int SI, class LdIn, class StOut>
-__device__ __forceinline__ void dft_rest(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in, StOut st_out)
+__device__ __forceinline__ void dft_rest(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in, StOut st_out, unsigned tid)
 {
     using Lay = LdsLayout<N, NT>;
     constexpr int P = Radix<N, MAXR>::of(NS);
     auto lds_ld = [&](int tr, int idx, int, int) { return lds[Lay::at(tr, idx)]; };
     auto lds_st = [&](int tr, int idx, double2 v, int, int) { lds[Lay::at(tr, idx)] = v; };
     if constexpr (NS * P >= N) {
-        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, IN_LDS && OUT_LDS>(t, ld_in, st_out);
-        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, OUT_LDS>(t, lds_ld, st_out);
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, IN_LDS && OUT_LDS>(t, ld_in, st_out, tid);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, OUT_LDS>(t, lds_ld, st_out, tid);
     } else {
-        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, IN_LDS>(t, ld_in, lds_st);
-        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, true>(t, lds_ld, lds_st);
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, IN_LDS>(t, ld_in, lds_st, tid);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, true>(t, lds_ld, lds_st, tid);
         __syncthreads();
-        dft_rest<N, NT, TPB, MAXR, INV, IN_LDS, OUT_LDS, NS * P, SI + 1>(lds, t, ld_in, st_out);
+        dft_rest<N, NT, TPB, MAXR, INV, IN_LDS, OUT_LDS, NS * P, SI + 1>(lds, t, ld_in, st_out, tid);
     }
 }
 
 template <int N, int NT, int TPB, bool INV, bool IN_LDS = false, bool OUT_LDS = false, int MAXR, class LdIn,
           class StOut>
-__device__ __forceinline__ void dft(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in, StOut st_out)
+__device__ __forceinline__ void dft(double2* lds, const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in, StOut st_out,
+                                    unsigned tid = threadIdx.x)
 {
-    dft_rest<N, NT, TPB, MAXR, INV, IN_LDS, OUT_LDS, 1, 0>(lds, t, ld_in, st_out);
+    dft_rest<N, NT, TPB, MAXR, INV, IN_LDS, OUT_LDS, 1, 0>(lds, t, ld_in, st_out, tid);
 }
 
 // Span and stage number of the last stage of an N-point transform.
@@ -405,20 +410,20 @@ struct Stages {
 
 template <int N, int NT, int TPB, int MAXR, bool INV, int NS, int SI, int DST, class LdIn, class StOut>
 __device__ __forceinline__ void dft_pp_rest(double2* const (&buf)[2], const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in,
-                                            StOut st_out)
+                                            StOut st_out, unsigned tid)
 {
     using Lay = LdsLayout<N, NT>;
     constexpr int P = Radix<N, MAXR>::of(NS);
     auto rd = [&](int tr, int idx, int, int) { return buf[1 - DST][Lay::at(tr, idx)]; };
     auto wr = [&](int tr, int idx, double2 v, int, int) { buf[DST][Lay::at(tr, idx)] = v; };
     if constexpr (NS * P >= N) {
-        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, st_out);
-        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, rd, st_out);
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, st_out, tid);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, rd, st_out, tid);
     } else {
-        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, wr);
-        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, rd, wr);
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, wr, tid);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, rd, wr, tid);
         __syncthreads();
-        dft_pp_rest<N, NT, TPB, MAXR, INV, NS * P, SI + 1, 1 - DST>(buf, t, ld_in, st_out);
+        dft_pp_rest<N, NT, TPB, MAXR, INV, NS * P, SI + 1, 1 - DST>(buf, t, ld_in, st_out, tid);
     }
 }
 
@@ -433,9 +438,9 @@ __device__ constexpr int pp_out()
 
 template <int N, int NT, int TPB, bool INV, int DST0, int MAXR, class LdIn, class StOut>
 __device__ __forceinline__ void dft_pp(double2* const (&buf)[2], const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in,
-                                       StOut st_out)
+                                       StOut st_out, unsigned tid = threadIdx.x)
 {
-    dft_pp_rest<N, NT, TPB, MAXR, INV, 1, 0, DST0>(buf, t, ld_in, st_out);
+    dft_pp_rest<N, NT, TPB, MAXR, INV, 1, 0, DST0>(buf, t, ld_in, st_out, tid);
 }
 
 // Every stage but the last, ping-pong; returns after a barrier with the data
@@ -447,17 +452,17 @@ __device__ constexpr int pp_head_out()
 }
 
 template <int N, int NT, int TPB, int MAXR, bool INV, int NS, int SI, int DST, class LdIn>
-__device__ __forceinline__ void dft_pp_head(double2* const (&buf)[2], const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in)
+__device__ __forceinline__ void dft_pp_head(double2* const (&buf)[2], const StageTw<N, NT, TPB, MAXR>& t, LdIn ld_in, unsigned tid)
 {
     using Lay = LdsLayout<N, NT>;
     constexpr int P = Radix<N, MAXR>::of(NS);
     if constexpr (NS * P < N) {
         auto rd = [&](int tr, int idx, int, int) { return buf[1 - DST][Lay::at(tr, idx)]; };
         auto wr = [&](int tr, int idx, double2 v, int, int) { buf[DST][Lay::at(tr, idx)] = v; };
-        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, wr);
-        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, rd, wr);
+        if constexpr (NS == 1) stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, ld_in, wr, tid);
+        else stage<N, NT, TPB, MAXR, P, NS, SI, INV, false>(t, rd, wr, tid);
         __syncthreads();
-        dft_pp_head<N, NT, TPB, MAXR, INV, NS * P, SI + 1, 1 - DST>(buf, t, ld_in);
+        dft_pp_head<N, NT, TPB, MAXR, INV, NS * P, SI + 1, 1 - DST>(buf, t, ld_in, tid);
     }
 }
 
@@ -469,21 +474,21 @@ __device__ __forceinline__ void dft_pp_head(double2* const (&buf)[2], const Stag
 // LDS round trip instead of two.
 template <int N, int NT, int TPB, int MAXR, class LdIn, class Pw, class StOut>
 __device__ __forceinline__ void dft_square_pass_pp(double2* const (&buf)[2], const StageTw<N, NT, TPB, MAXR>& t,
-                                                   LdIn ld_in, Pw pw, StOut st_out)
+                                                   LdIn ld_in, Pw pw, StOut st_out, unsigned tid = threadIdx.x)
 {
     using Lay = LdsLayout<N, NT>;
     constexpr int P0 = Radix<N, MAXR>::first();
     if constexpr (P0 == Radix<N, MAXR>::last() && LastStage<N, MAXR>::ns() > 1) {
         constexpr int H = pp_head_out<N, MAXR, 0>();  // the forward head's output
-        dft_pp_head<N, NT, TPB, MAXR, false, 1, 0, 0>(buf, t, ld_in);
+        dft_pp_head<N, NT, TPB, MAXR, false, 1, 0, 0>(buf, t, ld_in, tid);
         // the turn: forward last stage from buf[H], inverse first stage into buf[1 - H]
         constexpr int SI = LastStage<N, MAXR>::si();
         using S = StageShape<N, NT, TPB, P0>;
-        const int tr = threadIdx.x % NT;
+        const int tr = tid % NT;
 #pragma unroll
         for (int u = 0; u < S::PER; ++u) {
-            if (S::live(u)) {
-                const int j = S::j(u);
+            if (S::live(u, tid)) {
+                const int j = S::j(u, tid);
                 double2 v[P0];
 #pragma unroll
                 for (int r = 0; r < P0; ++r) v[r] = buf[H][Lay::at(tr, j + r * S::NBT)];
@@ -500,14 +505,14 @@ __device__ __forceinline__ void dft_square_pass_pp(double2* const (&buf)[2], con
             }
         }
         __syncthreads();
-        dft_pp_rest<N, NT, TPB, MAXR, true, P0, 1, H>(buf, t, ld_in, st_out);
+        dft_pp_rest<N, NT, TPB, MAXR, true, P0, 1, H>(buf, t, ld_in, st_out, tid);
     } else {
         constexpr int F = pp_out<N, MAXR, 0>();  // where the forward pass leaves the spectrum
-        dft_pp<N, NT, TPB, false, 0>(buf, t, ld_in,
-                                     [&](int tr, int idx, double2 X, int, int) { buf[F][Lay::at(tr, idx)] = pw(X); });
+        dft_pp<N, NT, TPB, false, 0>(
+            buf, t, ld_in, [&](int tr, int idx, double2 X, int, int) { buf[F][Lay::at(tr, idx)] = pw(X); }, tid);
         __syncthreads();
-        dft_pp<N, NT, TPB, true, 1 - F>(buf, t, [&](int tr, int idx, int, int) { return buf[F][Lay::at(tr, idx)]; },
-                                        st_out);
+        dft_pp<N, NT, TPB, true, 1 - F>(
+            buf, t, [&](int tr, int idx, int, int) { return buf[F][Lay::at(tr, idx)]; }, st_out, tid);
     }
 }
 

@@ -45,15 +45,18 @@ def test_square_exact_vs_python_int(gpu, k, n, L):
             assert 0 <= e < 0.25
 
 
-# Outer pass modes: "turn" (default: one launch with the carry hand-off
-# between column blocks; 2 kernels per squaring), "split" (PL_CARRY_TURN=0:
-# two launches; 3 kernels), "unfused" (PL_FUSED=0: 5 kernels).
-@pytest.mark.parametrize("mode", ["turn", "split", "unfused"])
+# Steady-state modes: "resident" (default: the whole steady state of a chunk
+# as one resident launch where the length has such a shape, else as
+# "turn"), "turn" (PL_RESIDENT=0: one launch for the outer pass with the
+# carry hand-off between column blocks, 2 kernels per squaring), "split"
+# (PL_CARRY_TURN=0: two launches; 3 kernels), "unfused" (PL_FUSED=0: 5).
+@pytest.mark.parametrize("mode", ["resident", "turn", "split", "unfused"])
 @pytest.mark.parametrize("k,n,L", [(3, 20000, 2048), (1048573, 60000, 4096), (11887, 300007, 16384),
                                    (433917, 2097152, 131072), (65535, 3000017, 262144)])
 def test_fused_and_unfused_paths(gpu, monkeypatch, mode, k, n, L):
     monkeypatch.setenv("PL_FUSED", "0" if mode == "unfused" else "1")
     monkeypatch.setenv("PL_CARRY_TURN", "0" if mode == "split" else "1")
+    monkeypatch.setenv("PL_RESIDENT", "1" if mode == "resident" else "0")
     rnd = random.Random(n)
     N = k * (1 << n) + 1
     x = rnd.randrange(N)
@@ -66,10 +69,12 @@ def test_fused_and_unfused_paths(gpu, monkeypatch, mode, k, n, L):
             assert c.info.passes == 5
         elif n / L >= 12:
             # runs of 16 digits (large k for the digit size) keep two launches
-            want = {"turn": (2, 3), "split": (3,)}[mode]
+            want = {"resident": (0, 2, 3), "turn": (2, 3), "split": (3,)}[mode]
             assert c.info.passes in want, "split outer pass expected for this length"
             if mode == "turn" and n >= 10 ** 6:
                 assert c.info.passes == 2
+            if mode == "resident" and L in (131072, 262144):
+                assert c.info.passes == 0
         c.set_residue(x)
         e, tr = c.square(iters, trace=True)
         assert 0 < e < 0.3 and len(tr) == iters

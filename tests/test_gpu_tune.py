@@ -42,7 +42,7 @@ def test_every_pinned_plan_squares_exactly(gpu, k, n):
         c.set_residue(x)
         c.square(SQUARINGS)
         want = c.get_residue()
-    assert rule[1:4] == (8, 8, 8) and rule[4] == 2
+    assert rule[1:4] == (8, 8, 8) and rule[4] in (0, 2)
     if n <= 1 << 21:
         assert want == G.powm(x, 1 << SQUARINGS, N)
     logL = L.bit_length() - 1
@@ -53,7 +53,7 @@ def test_every_pinned_plan_squares_exactly(gpu, k, n):
             i = c.info
             case = (k, n, rows, rr, cr, run, one)
             assert (i.rows, i.cols, i.row_radix, i.col_radix, i.run) == (rows, L // rows, rr, cr, run), case
-            assert i.passes == (2 if one else 3), case
+            assert i.passes in (((0, 2) if rr == cr == 8 else (2,)) if one else (3,)), case
             c.set_residue(x)
             e = c.square(SQUARINGS)
             assert c.get_residue() == want, case
@@ -76,7 +76,7 @@ def test_autotune_plan_records_an_exact_fastest_choice(gpu):
         i = c.info
         assert (i.rows, i.row_radix, i.col_radix, i.run) == (best["rows"], best["row_radix"], best["col_radix"],
                                                              best["run"])
-        assert i.passes == (2 if best["one_launch"] else 3)
+        assert i.passes in ((0, 2) if best["one_launch"] else (3,))
         N = k * (1 << n) + 1
         x = random.Random(3).randrange(2, N - 1)
         c.set_residue(x)
