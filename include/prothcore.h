@@ -197,6 +197,14 @@ int pl_fft_square_karatsuba_noalloc(int device, uint64_t* v, size_t n_limbs, uns
  * each full-array kernel moves (read + write of the L complex points). */
 int pl_kernel_times(pl_ctx* ctx, int reps, double* ms, uint64_t* traffic_bytes);
 
+/* Diagnostics (no counterpart in the reference): device time of the resident
+ * launches (pl_info.passes == 0) that the context's pl_square calls ran, as
+ * they ran: HIP events around each launch on the context's stream, summed
+ * over completed chunks.  *ms (nullable) gets the total, *launches and
+ * *squarings (nullable) how many launches and how many squarings they
+ * carried; reset != 0 clears the sums after reading them. */
+int pl_loop_time(pl_ctx* ctx, int reset, double* ms, uint64_t* launches, uint64_t* squarings);
+
 /* Diagnostics (no counterpart in the reference): *ms <- average duration of
  * a plain device-to-device copy of `bytes` bytes (reads and writes `bytes`
  * each) over `reps` launches on `device`.  The measured streaming rate of

@@ -104,6 +104,7 @@ def lib() -> ctypes.CDLL:
         "pl_fft_square_karatsuba_noalloc": ([i32, _u64p, sz, ctypes.c_uint, u64, _dp], i32),
         "pl_kernel_times": ([vp, i32, _dp, _u64p], i32),
         "pl_copy_probe": ([i32, u64, i32, _dp], i32),
+        "pl_loop_time": ([vp, i32, _dp, _u64p, _u64p], i32),
         "pl_autotune_fft_len": ([u32, u64, i32, dbl, ctypes.POINTER(u32), _dp, _dp], i32),
         "pl_autotune_plan": ([i32, u64, u32, ctypes.POINTER(TuneTrial), u32, ctypes.POINTER(u32),
                               ctypes.POINTER(TuneTrial)], i32),
@@ -260,6 +261,14 @@ class Context:
         tb = ctypes.c_uint64(0)
         _check(lib().pl_kernel_times(self.h, reps, ms, ctypes.byref(tb)))
         return dict(zip(self.KERNELS, list(ms))), tb.value
+
+    def loop_time(self, reset: bool = False) -> dict:
+        """Device time of the resident launches the context's squarings ran
+        (HIP events around each launch), with launch and squaring counts."""
+        ms = ctypes.c_double(0.0)
+        nl, ns = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(lib().pl_loop_time(self.h, 1 if reset else 0, ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(ns)))
+        return dict(ms=ms.value, launches=nl.value, squarings=ns.value)
 
     def fft_square_karatsuba(self, limbs, n_limbs: int):
         arr = (ctypes.c_uint64 * n_limbs)(*limbs)

@@ -384,6 +384,15 @@ int pl_kernel_times(pl_ctx* ctx, int reps, double* ms, uint64_t* traffic_bytes)
     });
 }
 
+int pl_loop_time(pl_ctx* ctx, int reset, double* ms, uint64_t* launches, uint64_t* squarings)
+{
+    if (!ctx) return fail(PL_ERR_ARG, "NULL context");
+    return guarded([&] {
+        ctx->eng->loop_time(reset != 0, ms, launches, squarings);
+        return PL_OK;
+    });
+}
+
 int pl_copy_probe(int device, uint64_t bytes, int reps, double* ms)
 {
     if (!ms || bytes < 16 || reps <= 0) return fail(PL_ERR_ARG, "bad argument");

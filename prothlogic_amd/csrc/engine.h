@@ -171,6 +171,10 @@ class Group {
     // carry hand-off gave up is recomputed from its saved start on the split
     // pass.  Number of such recomputations so far:
     uint32_t handoff_fallbacks() const { return fallbacks_; }
+    // Device time of the resident launches of completed plain chunks (HIP
+    // events around each launch, on this group's stream), how many launches
+    // and how many squarings they ran; reset clears the sums after reading.
+    void loop_time(bool reset, double* ms, uint64_t* launches, uint64_t* squarings);
 
     // Average duration (ms) of each kernel over all slots, timed with HIP
     // events between consecutive kernels of `reps` real plain squarings on
@@ -202,6 +206,7 @@ class Group {
     void build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g, bool use_turn);
     void run_plain_chunk(int g, bool use_turn);
     void ensure_backup();
+    void ensure_loop_events();
     void restore_start(uint32_t slot0, uint32_t slots);
     void sync_status(const char* what);
     double read_err(int g, double* trace, double* slot_err, uint32_t trace_slot = 0, uint32_t nslots = 0);
@@ -255,6 +260,9 @@ class Group {
     // and must not see graph replays)
     bool no_graphs_ = false;
     uint32_t fallbacks_ = 0;
+    hipEvent_t lev_[2] = { nullptr, nullptr };  // around the resident launch of a chunk
+    double loop_ms_ = 0;
+    uint64_t loop_launches_ = 0, loop_squarings_ = 0;
     bool owns_ = true;     // the memory and the stream are the group's own
     // the status word as read at the last wait on the stream, and whether
     // kernels were launched since

@@ -1836,6 +1836,8 @@ Group::~Group()
 {
     // nothing launched on the stream may still use the graphs or the memory
     (void)hipStreamSynchronize(stream_);
+    for (hipEvent_t& e : lev_)
+        if (e) (void)hipEventDestroy(e);
     for (auto& kv : plain_graphs_) {
         (void)hipGraphExecDestroy(kv.second.first);
         (void)hipGraphDestroy(kv.second.second);
@@ -2017,7 +2019,10 @@ void Group::enqueue_plain_chunk(int g, bool use_turn)
         Params pt = params(s0, false);
         pt.runc = runt_;
         launched_ = true;
+        ensure_loop_events();
+        check(hipEventRecord(lev_[0], stream_), "hipEventRecord");
         launch_loop(loop_var_, pt, (unsigned)(g - 1), xw_ + s0, stream_);
+        check(hipEventRecord(lev_[1], stream_), "hipEventRecord");
         enqueue_exit(params(s0, false), ns);
         return;
     }
@@ -2142,7 +2147,11 @@ double Group::square(uint64_t iters, double* err_trace) { return square_slots(0,
 // steady squarings, exit -- measured slower on MI355X than the launches.)
 void Group::run_plain_chunk(int g, bool use_turn)
 {
-    if (g == kChunk && !no_graphs_) {
+    // A resident chunk is a handful of launches around one long one, so it
+    // gains nothing from a graph, and the events around its resident launch
+    // must be recorded on the stream itself to be timed.
+    const bool resident = use_turn && g >= 2 && resident_on();
+    if (g == kChunk && !no_graphs_ && !resident) {
         const uint64_t key = ((uint64_t)sq0_ << 33) | ((uint64_t)sqn_ << 1) | (use_turn ? 1u : 0u);
         auto it = plain_graphs_.find(key);
         if (it == plain_graphs_.end()) {
@@ -2173,6 +2182,26 @@ void Group::ensure_backup()
     xbak_ = reinterpret_cast<long long*>(static_cast<char*>(m) + sizeof(double2) * plan_.L * cap_ + 256);
 }
 
+// Events around each resident launch (created before any graph that records
+// them is captured; no system-scope fence, which would add a cache
+// write-back the chunk itself does not do).
+void Group::ensure_loop_events()
+{
+    for (hipEvent_t& e : lev_)
+        if (!e) check(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
+}
+
+void Group::loop_time(bool reset, double* ms, uint64_t* launches, uint64_t* squarings)
+{
+    if (ms) *ms = loop_ms_;
+    if (launches) *launches = loop_launches_;
+    if (squarings) *squarings = loop_squarings_;
+    if (reset) {
+        loop_ms_ = 0;
+        loop_launches_ = loop_squarings_ = 0;
+    }
+}
+
 void Group::restore_start(uint32_t slot0, uint32_t slots)
 {
     check(hipMemcpyAsync(z_ + (size_t)slot0 * plan_.L, zbak_, sizeof(double2) * plan_.L * slots,
@@ -2200,6 +2229,14 @@ double Group::square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, doubl
             ensure_backup();
             run_plain_chunk(g, true);
             double e = read_err(g, tr, nullptr, slot0);
+            if (g >= 2 && resident_on() && !(seen_status_ & 2)) {
+                // the resident launch's own duration, from the events around it
+                float ms = 0;
+                check(hipEventElapsedTime(&ms, lev_[0], lev_[1]), "hipEventElapsedTime");
+                loop_ms_ += ms;
+                ++loop_launches_;
+                loop_squarings_ += (uint64_t)(g - 1);
+            }
             if (seen_status_ & 2) {
                 // a hand-off wait gave up (another process's kernels held the
                 // CUs its predecessor needed, or the test hook): the chunk's
