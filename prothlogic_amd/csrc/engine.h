@@ -200,7 +200,10 @@ class Group {
     void init(void* ext_mem, size_t ext_bytes, hipStream_t ext_stream);
     Params params(uint32_t slot0, bool check_active) const;
     void launch(int kind, const Params& p, uint32_t slots);
-    void enqueue_plain_chunk(int g, bool use_turn);
+    // parity >= 0: a pipelined resident chunk (its backup and events)
+    void enqueue_plain_chunk(int g, bool use_turn, int parity = -1);
+    uint64_t square_pipelined(uint64_t iters, double* err_trace, double& emax);
+    void ensure_pipeline();
     void enqueue_exit(const Params& p, uint32_t slots);
     void enqueue_step_chunk(int g);
     void build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g, bool use_turn);
@@ -208,6 +211,7 @@ class Group {
     void ensure_backup();
     void ensure_loop_events();
     void restore_start(uint32_t slot0, uint32_t slots);
+    void skip_epochs(uint32_t slot);
     void sync_status(const char* what);
     double read_err(int g, double* trace, double* slot_err, uint32_t trace_slot = 0, uint32_t nslots = 0);
     void enqueue_err_clear(uint32_t slots, unsigned col0);
@@ -260,7 +264,17 @@ class Group {
     // and must not see graph replays)
     bool no_graphs_ = false;
     uint32_t fallbacks_ = 0;
-    hipEvent_t lev_[2] = { nullptr, nullptr };  // around the resident launch of a chunk
+    hipEvent_t lev_[2][2] = { { nullptr, nullptr }, { nullptr, nullptr } };  // around a chunk's resident launch,
+                                                                             // per pipeline parity
+    // pipelined resident chunks (square_pipelined): per parity a one-slot
+    // backup of the chunk's start, pinned round-off columns + status word,
+    // and the chunk's end event
+    void* pipe_mem_ = nullptr;
+    double2* pipe_z_[2] = { nullptr, nullptr };
+    long long* pipe_x_[2] = { nullptr, nullptr };
+    unsigned long long* pipe_err_[2] = { nullptr, nullptr };
+    int* pipe_st_[2] = { nullptr, nullptr };
+    hipEvent_t pipe_done_[2] = { nullptr, nullptr };
     double loop_ms_ = 0;
     uint64_t loop_launches_ = 0, loop_squarings_ = 0;
     bool owns_ = true;     // the memory and the stream are the group's own

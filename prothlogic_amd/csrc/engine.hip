@@ -1836,8 +1836,14 @@ Group::~Group()
 {
     // nothing launched on the stream may still use the graphs or the memory
     (void)hipStreamSynchronize(stream_);
-    for (hipEvent_t& e : lev_)
-        if (e) (void)hipEventDestroy(e);
+    for (auto& pr : lev_)
+        for (hipEvent_t& e : pr)
+            if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; ++i) {
+        if (pipe_done_[i]) (void)hipEventDestroy(pipe_done_[i]);
+        if (pipe_err_[i]) (void)hipHostFree(pipe_err_[i]);
+    }
+    if (pipe_mem_) (void)hipFree(pipe_mem_);
     for (auto& kv : plain_graphs_) {
         (void)hipGraphExecDestroy(kv.second.first);
         (void)hipGraphDestroy(kv.second.second);
@@ -2000,14 +2006,16 @@ void Group::download(uint32_t slot, std::vector<int32_t>& digits, int64_t& xhi)
 // g squarings of every slot from the digit state back to the digit state;
 // iteration i's round-off lands in err column i.
 // g plain squarings of slots [sq0_, sq0_ + sqn_).
-void Group::enqueue_plain_chunk(int g, bool use_turn)
+void Group::enqueue_plain_chunk(int g, bool use_turn, int parity)
 {
     const uint32_t s0 = sq0_, ns = sqn_;
     if (use_turn) {
         const size_t count = (size_t)plan_.L * ns;
         const unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 2048);
-        hipLaunchKernelGGL(k_backup, dim3(blocks), dim3(256), 0, stream_, z_ + (size_t)s0 * plan_.L, zbak_, count,
-                           xhi_ + s0, xbak_, ns);
+        double2* bz = parity < 0 ? zbak_ : pipe_z_[parity];
+        long long* bx = parity < 0 ? xbak_ : pipe_x_[parity];
+        hipLaunchKernelGGL(k_backup, dim3(blocks), dim3(256), 0, stream_, z_ + (size_t)s0 * plan_.L, bz, count,
+                           xhi_ + s0, bx, ns);
     }
     launch(kFwdCol, params(s0, false), ns);
     if (use_turn && g >= 2 && resident_on()) {
@@ -2020,9 +2028,10 @@ void Group::enqueue_plain_chunk(int g, bool use_turn)
         pt.runc = runt_;
         launched_ = true;
         ensure_loop_events();
-        check(hipEventRecord(lev_[0], stream_), "hipEventRecord");
+        const int ev = parity < 0 ? 0 : parity;
+        check(hipEventRecord(lev_[ev][0], stream_), "hipEventRecord");
         launch_loop(loop_var_, pt, (unsigned)(g - 1), xw_ + s0, stream_);
-        check(hipEventRecord(lev_[1], stream_), "hipEventRecord");
+        check(hipEventRecord(lev_[ev][1], stream_), "hipEventRecord");
         enqueue_exit(params(s0, false), ns);
         return;
     }
@@ -2187,8 +2196,29 @@ void Group::ensure_backup()
 // write-back the chunk itself does not do).
 void Group::ensure_loop_events()
 {
-    for (hipEvent_t& e : lev_)
-        if (!e) check(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
+    for (auto& pr : lev_)
+        for (hipEvent_t& e : pr)
+            if (!e) check(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
+}
+
+// Buffers of the pipelined resident chunks: per parity a backup of the
+// chunk's start (one slot), a pinned host copy of its round-off columns and
+// status word, and an event marking its end.
+void Group::ensure_pipeline()
+{
+    ensure_loop_events();
+    if (pipe_mem_) return;
+    const size_t zb = sizeof(double2) * plan_.L, per = zb + 256;
+    check(hipMalloc(&pipe_mem_, 2 * per), "hipMalloc pipeline backup");
+    for (int i = 0; i < 2; ++i) {
+        char* b = static_cast<char*>(pipe_mem_) + (size_t)i * per;
+        pipe_z_[i] = reinterpret_cast<double2*>(b);
+        pipe_x_[i] = reinterpret_cast<long long*>(b + zb);
+        check(hipHostMalloc(reinterpret_cast<void**>(&pipe_err_[i]), sizeof(unsigned long long) * kChunk * cap_ + 64),
+              "hipHostMalloc");
+        pipe_st_[i] = reinterpret_cast<int*>(pipe_err_[i] + (size_t)kChunk * cap_);
+        check(hipEventCreateWithFlags(&pipe_done_[i], hipEventDisableTiming), "hipEventCreate");
+    }
 }
 
 void Group::loop_time(bool reset, double* ms, uint64_t* launches, uint64_t* squarings)
@@ -2202,6 +2232,23 @@ void Group::loop_time(bool reset, double* ms, uint64_t* launches, uint64_t* squa
     }
 }
 
+// After a launch whose hand-off gave up: the tagged words may hold tags of
+// its unfinished range, which the next launch would reuse (a resident launch
+// advances the slot's epoch only when it completes).  Moving the epoch 256
+// on puts the next launches' tags clear of every tag a word can hold (the
+// last completed chunk's and the failed range, each under 256 wide).
+void Group::skip_epochs(uint32_t slot)
+{
+    unsigned ep[4];
+    check(hipMemcpyAsync(ep, epoch_ + 4 * (size_t)slot, sizeof ep, hipMemcpyDeviceToHost, stream_), "epoch D2H");
+    check(hipStreamSynchronize(stream_), "epoch sync");
+    const unsigned e = (ep[0] + (unsigned)kChunk) & kTagMask;
+    ep[1] = (e + kTagMask) & kTagMask;
+    ep[0] = e;
+    check(hipMemcpyAsync(epoch_ + 4 * (size_t)slot, ep, sizeof ep, hipMemcpyHostToDevice, stream_), "epoch H2D");
+    check(hipStreamSynchronize(stream_), "epoch sync");
+}
+
 void Group::restore_start(uint32_t slot0, uint32_t slots)
 {
     check(hipMemcpyAsync(z_ + (size_t)slot0 * plan_.L, zbak_, sizeof(double2) * plan_.L * slots,
@@ -2209,6 +2256,84 @@ void Group::restore_start(uint32_t slot0, uint32_t slots)
           "restore z");
     check(hipMemcpyAsync(xhi_ + slot0, xbak_, sizeof(long long) * slots, hipMemcpyDeviceToDevice, stream_),
           "restore xhi");
+}
+
+// Whole resident chunks of slot sq0_, pipelined: chunk i+1 is enqueued
+// before chunk i is waited for, so the device never waits for the host
+// between chunks.  Each chunk saves its start in its parity's backup and
+// copies its round-off columns and status word to its parity's pinned
+// buffers.  Runs chunks of at least 2 squarings while `iters - done` allows
+// and the token is held by the caller; returns the squarings done (the
+// caller finishes any rest).  A chunk whose hand-off gave up is recomputed
+// from its start on the split pass, and the chunk after it is discarded
+// and left to the caller.
+uint64_t Group::square_pipelined(uint64_t iters, double* err_trace, double& emax)
+{
+    ensure_pipeline();
+    const uint32_t s0 = sq0_;
+    uint64_t done = 0, queued = 0;
+    int inflight[2] = { 0, 0 };  // squarings of the chunk in flight per parity (0: none)
+    uint64_t start[2] = { 0, 0 };
+    int next = 0;
+    auto enqueue = [&](int par, int g) {
+        enqueue_err_clear(cap_, ~0u);
+        enqueue_plain_chunk(g, true, par);
+        enqueue_err_reduce(0, cap_);
+        check(hipGetLastError(), "square launch");
+        check(hipMemcpyAsync(pipe_err_[par], err_, sizeof(unsigned long long) * kChunk * cap_, hipMemcpyDeviceToHost,
+                             stream_),
+              "err D2H");
+        check(hipMemcpyAsync(pipe_st_[par], status_, sizeof(int), hipMemcpyDeviceToHost, stream_), "status D2H");
+        check(hipEventRecord(pipe_done_[par], stream_), "hipEventRecord");
+        launched_ = true;
+        inflight[par] = g;
+        start[par] = queued;
+        queued += (uint64_t)g;
+    };
+    auto room = [&]() { return iters - queued >= 2 ? (int)std::min<uint64_t>(kChunk, iters - queued) : 0; };
+    if (const int g = room()) enqueue(next, g);
+    while (inflight[next]) {
+        const int par = next, other = 1 - next;
+        if (!inflight[other])
+            if (const int g = room()) enqueue(other, g);
+        check(hipEventSynchronize(pipe_done_[par]), "chunk sync");
+        const int g = inflight[par];
+        inflight[par] = 0;
+        const int st = *pipe_st_[par];
+        if (st & 2) {
+            // a hand-off wait gave up: drain, recompute this chunk from its
+            // start on the split pass; the chunk after it is void
+            check(hipStreamSynchronize(stream_), "chunk sync");
+            clear_status();
+            check(hipMemcpyAsync(z_ + (size_t)s0 * plan_.L, pipe_z_[par], sizeof(double2) * plan_.L,
+                                 hipMemcpyDeviceToDevice, stream_),
+                  "restore z");
+            check(hipMemcpyAsync(xhi_ + s0, pipe_x_[par], sizeof(long long), hipMemcpyDeviceToDevice, stream_),
+                  "restore xhi");
+            skip_epochs(s0);
+            run_plain_chunk(g, false);
+            double* tr = err_trace ? err_trace + start[par] : nullptr;
+            emax = std::max(emax, read_err(g, tr, nullptr, s0));
+            ++fallbacks_;
+            return start[par] + (uint64_t)g;
+        }
+        seen_status_ |= st;
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, lev_[par][0], lev_[par][1]), "hipEventElapsedTime");
+        loop_ms_ += ms;
+        ++loop_launches_;
+        loop_squarings_ += (uint64_t)(g - 1);
+        for (int i = 0; i < g; ++i) {
+            double e;
+            std::memcpy(&e, &pipe_err_[par][(size_t)s0 * kChunk + i], sizeof e);
+            if (err_trace) err_trace[start[par] + (uint64_t)i] = e;
+            emax = std::max(emax, e);
+        }
+        done = start[par] + (uint64_t)g;
+        next = other;
+    }
+    launched_ = false;
+    return done;
 }
 
 double Group::square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, double* err_trace)
@@ -2219,6 +2344,10 @@ double Group::square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, doubl
     sqn_ = slots;
     double emax = 0.0;
     uint64_t done = 0;
+    if (iters >= 2 && resident_on()) {
+        TurnToken token(device_, true);
+        if (token.held) done = square_pipelined(iters, err_trace, emax);
+    }
     while (done < iters) {
         const int g = (int)std::min<uint64_t>(kChunk, iters - done);
         double* tr = err_trace ? err_trace + done : nullptr;
@@ -2232,7 +2361,7 @@ double Group::square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, doubl
             if (g >= 2 && resident_on() && !(seen_status_ & 2)) {
                 // the resident launch's own duration, from the events around it
                 float ms = 0;
-                check(hipEventElapsedTime(&ms, lev_[0], lev_[1]), "hipEventElapsedTime");
+                check(hipEventElapsedTime(&ms, lev_[0][0], lev_[0][1]), "hipEventElapsedTime");
                 loop_ms_ += ms;
                 ++loop_launches_;
                 loop_squarings_ += (uint64_t)(g - 1);
@@ -2243,6 +2372,7 @@ double Group::square_slots(uint32_t slot0, uint32_t slots, uint64_t iters, doubl
                 // result is void; recompute it from its start on the split pass
                 clear_status();
                 restore_start(slot0, slots);
+                for (uint32_t i = 0; i < slots; ++i) skip_epochs(slot0 + i);
                 run_plain_chunk(g, false);
                 e = read_err(g, tr, nullptr, slot0);
                 ++fallbacks_;
