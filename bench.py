@@ -452,6 +452,70 @@ def batch_c4_full(core, workloads, out_path=None):
                 max_err=max(r["max_err"] for r in res), tuned_plans=tuned)
 
 
+def serve_c5_full(core, workloads, seg, nseg, rate, out_path=None, seed=5):
+    """Config 5 at its stated size, one segment of it: the 2000 candidates
+    (k odd in [3,2^20), n log-uniform in [5e4,4e6], PCG64 seed 5), every test
+    run to its verdict, arriving as one Poisson process of `rate` tests per
+    second (exponential gaps, PCG64(seed)).  The whole run does not fit one
+    bounded GPU session, so it is cut into `nseg` consecutive segments of the
+    arrival sequence; segment `seg` replays its arrivals (times shifted to
+    start at 0) into a fresh pool.  Segment 0 also receives the known prime
+    843*2^120002+1 right after its first arrival.  Every test's (k, n,
+    arrival, time to verdict, verdict, Res64, max round-off) goes to
+    out_path; the segments are summarised together afterwards."""
+    import numpy as np
+    w = workloads.c5()
+    cands = list(w.candidates)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(cands)))
+    lo, hi = seg * len(cands) // nseg, (seg + 1) * len(cands) // nseg
+    todo = [(i, cands[i][0], cands[i][1], float(arrivals[i] - arrivals[lo])) for i in range(lo, hi)]
+    if seg == 0:
+        todo.insert(1, (len(cands), 843, 120002, todo[1][3]))
+    meta = {t[0]: t for t in todo}
+    out = {}
+    with core.Pool(k_max=1 << 20) as pool:
+        tuned = tune_batch(core, pool, [(k, n) for _, k, n, _ in todo])
+        log(f"[bench] c5 full segment {seg}/{nseg}: {len(todo)} arrivals at {rate}/s, tuned {len(tuned.get('pool_groups', {}))} groups")
+        t0 = time.perf_counter()
+        nxt, last_log = 0, t0
+        while len(out) < len(todo):
+            now = time.perf_counter() - t0
+            while nxt < len(todo) and todo[nxt][3] <= now:
+                tag, k, n, _ = todo[nxt]
+                pool.submit(k, n, tag=tag)
+                nxt += 1
+            st = pool.status()
+            if st["active"] == 0 and st["queued"] == 0 and nxt < len(todo):
+                time.sleep(max(0.0, min(0.01, todo[nxt][3] - now)))
+            for r in pool.run(steps=64):
+                out[r["tag"]] = r
+            if time.perf_counter() - last_log > 30:
+                last_log = time.perf_counter()
+                log(f"[bench] c5 full segment {seg}: {now:.0f} s, {nxt}/{len(todo)} arrived, {len(out)} verdicts, "
+                    f"{st['active']} active, {st['queued']} queued")
+        makespan = time.perf_counter() - t0
+    rows = []
+    for tag, k, n, arr in todo:
+        r = out[tag]
+        rows.append(dict(tag=tag, k=k, n=n, arrival_s=round(arr, 4), latency_s=round(r["t_done"] - r["t_submit"], 4),
+                         verdict=r["verdict"], error=r["error"], res64=format(r["res64"], "016X"),
+                         max_err=r["max_err"]))
+    if out_path:
+        Path(out_path).write_text(json.dumps(dict(segment=seg, segments=nseg, rate=rate, makespan_s=makespan,
+                                                  tuned_plans=tuned, tests=rows)))
+    lat = np.array([r["latency_s"] for r in rows])
+    return dict(workload=f"c5_serving (full, segment {seg} of {nseg}): arrivals {lo}..{hi - 1} of 2000"
+                         + (" + 843*2^120002+1" if seg == 0 else ""),
+                rate_per_s=rate, tests=len(rows), makespan_s=round(makespan, 2),
+                p50_time_to_verdict_s=round(float(np.percentile(lat, 50)), 3),
+                p99_time_to_verdict_s=round(float(np.percentile(lat, 99)), 3),
+                verdicts={"prime": sum(1 for r in rows if r["verdict"] == 1),
+                          "composite": sum(1 for r in rows if r["verdict"] == 0),
+                          "void": sum(1 for r in rows if r["error"] != 0)},
+                max_err=max(r["max_err"] for r in rows))
+
+
 def serve_c5(core, workloads, count=200, scale=100, load=2.0, seed=5):
     """Config 5 (scaled): Poisson arrivals of the first `count` config-5
     candidates (k odd in [3,2^20), n log-uniform in [5e4,4e6]), each test cut
@@ -521,6 +585,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) leg")
     ap.add_argument("--c4-full", metavar="OUT.json", default=None,
                     help="only run config 4 at its stated size (every test to its verdict); per-test results to OUT")
+    ap.add_argument("--c5-full", nargs=4, metavar=("SEG", "NSEG", "RATE", "OUT.json"), default=None,
+                    help="only run segment SEG of NSEG of config 5 at its stated size (Poisson arrivals at RATE "
+                         "tests/s, every test to its verdict); per-test results to OUT")
     ap.add_argument("--cpu-only", action="store_true", help="only the CPU legs (no GPU needed)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-plan autotuner (plans from --plans, else the built-in rule); for "
@@ -563,6 +630,10 @@ def main():
         core.tune_load(args.plans)
     if args.c4_full:
         print(json.dumps(batch_c4_full(core, workloads, args.c4_full)), flush=True)
+        return
+    if args.c5_full:
+        seg, nseg, rate, outp = args.c5_full
+        print(json.dumps(serve_c5_full(core, workloads, int(seg), int(nseg), float(rate), outp)), flush=True)
         return
 
     log(f"[bench] c2: k={k2} n={n2} steps={args.steps} warmup={args.warmup}")
