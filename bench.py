@@ -488,8 +488,16 @@ def serve_c5_full(core, workloads, seg, nseg, rate, out_path=None, seed=5):
             st = pool.status()
             if st["active"] == 0 and st["queued"] == 0 and nxt < len(todo):
                 time.sleep(max(0.0, min(0.01, todo[nxt][3] - now)))
-            for r in pool.run(steps=64):
+            got = pool.run(steps=64)
+            for r in got:
                 out[r["tag"]] = r
+            if got and out_path:
+                # results so far, so that a run cut short still leaves them
+                with open(out_path + ".partial", "a") as f:
+                    for r in got:
+                        f.write(json.dumps(dict(tag=r["tag"], k=r["k"], n=r["n"], latency_s=r["t_done"] - r["t_submit"],
+                                                verdict=r["verdict"], error=r["error"],
+                                                res64=format(r["res64"], "016X"), max_err=r["max_err"])) + "\n")
             if time.perf_counter() - last_log > 30:
                 last_log = time.perf_counter()
                 log(f"[bench] c5 full segment {seg}: {now:.0f} s, {nxt}/{len(todo)} arrived, {len(out)} verdicts, "
@@ -504,6 +512,17 @@ def serve_c5_full(core, workloads, seg, nseg, rate, out_path=None, seed=5):
     if out_path:
         Path(out_path).write_text(json.dumps(dict(segment=seg, segments=nseg, rate=rate, makespan_s=makespan,
                                                   tuned_plans=tuned, tests=rows)))
+    # the plain one-at-a-time loop's steady time per squaring at each length
+    # of the segment (untimed against the serving run, after it)
+    plain = {}
+    for _, k, n, _ in todo:
+        L = core.plan_fft_len(n, k)
+        if L not in plain:
+            plain[L] = plain_per_squaring(core, k, n, iters=768)[0]
+    if out_path:
+        d = json.loads(Path(out_path).read_text())
+        d["plain_s_per_squaring_by_fft_len"] = {str(L): v for L, v in sorted(plain.items())}
+        Path(out_path).write_text(json.dumps(d))
     lat = np.array([r["latency_s"] for r in rows])
     return dict(workload=f"c5_serving (full, segment {seg} of {nseg}): arrivals {lo}..{hi - 1} of 2000"
                          + (" + 843*2^120002+1" if seg == 0 else ""),
@@ -513,7 +532,8 @@ def serve_c5_full(core, workloads, seg, nseg, rate, out_path=None, seed=5):
                 verdicts={"prime": sum(1 for r in rows if r["verdict"] == 1),
                           "composite": sum(1 for r in rows if r["verdict"] == 0),
                           "void": sum(1 for r in rows if r["error"] != 0)},
-                max_err=max(r["max_err"] for r in rows))
+                max_err=max(r["max_err"] for r in rows),
+                plain_us_per_squaring_by_fft_len={str(L): round(v * 1e6, 2) for L, v in sorted(plain.items())})
 
 
 def serve_c5(core, workloads, count=200, scale=100, load=2.0, seed=5):

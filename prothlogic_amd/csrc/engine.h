@@ -127,6 +127,9 @@ class Group {
     // for those slots only (0 for the rest).
     void step_begin(uint32_t g, uint32_t slots = 0);
     double step_end(double* slot_err);
+    // Whether the chunk step_begin enqueued has finished (step_end then
+    // returns without waiting); true when none is in flight.
+    bool step_ready();
     double leave(uint32_t slot);
     std::vector<SlotState> states();
 

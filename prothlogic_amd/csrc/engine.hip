@@ -2467,6 +2467,15 @@ void Group::step_begin(uint32_t g, uint32_t slots)
     pending_steps_ = g;
 }
 
+bool Group::step_ready()
+{
+    if (!pending_steps_) return true;
+    const hipError_t e = hipStreamQuery(stream_);
+    if (e == hipErrorNotReady) return false;
+    check(e, "hipStreamQuery");
+    return true;
+}
+
 double Group::step_end(double* slot_err)
 {
     if (!pending_steps_) throw std::logic_error("step_end: no chunk in flight");

@@ -18,6 +18,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "bignum.h"
@@ -419,17 +420,51 @@ int pl_pool_run(pl_pool* P, uint32_t steps, pl_pool_result* out, size_t cap, siz
                 span.push_back(hi);
             }
         }
-        for (uint32_t done = 0; done < steps;) {
-            const uint32_t g = std::min<uint32_t>(pleng::Group::kStepChunk, steps - done);
-            for (size_t i = 0; i < live.size(); ++i) live[i]->g->step_begin(g, span[i]);
-            for (LenGroup* lg : live) {
+        // Groups step independently: each enqueues its next chunk as soon as
+        // its last one is read back, so a group of short transforms is not
+        // held to the pace of a group of long ones.  The call returns once
+        // every group has stepped at least `steps` times; groups that get
+        // there first keep stepping meanwhile (their tests keep advancing).
+        std::vector<uint32_t> done(live.size(), 0);
+        std::vector<uint32_t> inflight(live.size(), 0);
+        auto behind = [&]() {
+            for (size_t i = 0; i < live.size(); ++i)
+                if (done[i] < steps) return true;
+            return false;
+        };
+        auto launch = [&](size_t i) {
+            const uint32_t want = done[i] < steps ? steps - done[i] : pleng::Group::kStepChunk;
+            const uint32_t g = std::min<uint32_t>(pleng::Group::kStepChunk, want);
+            live[i]->g->step_begin(g, span[i]);
+            inflight[i] = g;
+        };
+        for (size_t i = 0; i < live.size(); ++i) launch(i);
+        for (;;) {
+            bool waiting = false;
+            for (size_t i = 0; i < live.size(); ++i) {
+                if (!inflight[i]) continue;
+                LenGroup* lg = live[i];
+                if (!lg->g->step_ready()) {
+                    waiting = true;
+                    continue;
+                }
                 // step_end reports every slot of the group, scratch slots included
                 std::vector<double> se(lg->g->capacity(), 0.0);
                 lg->g->step_end(se.data());
                 for (size_t s = 0; s < lg->slots.size(); ++s)
                     if (lg->slots[s].used) lg->slots[s].max_err = std::max(lg->slots[s].max_err, se[s]);
+                done[i] += inflight[i];
+                inflight[i] = 0;
+                if (lg->g->status()) {
+                    done[i] = std::max(done[i], steps);  // evicted below; no more steps on a void state
+                    continue;
+                }
+                if (behind()) launch(i);
             }
-            done += g;
+            bool any = false;
+            for (uint32_t f : inflight) any = any || f != 0;
+            if (!any) break;
+            if (waiting) std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
         // A group whose carry did not settle in these steps holds no reliable
         // residue in any slot: its tests restart from their seeds.

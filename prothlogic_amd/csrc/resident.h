@@ -224,11 +224,14 @@ __device__ __forceinline__ bool loop_outer(const Params& p, double2* const (&buf
         if (q >= (unsigned)RUNS) break;
         const unsigned b = w * BPT + sub;
         const unsigned F = (unsigned)((0ull - p.n * (unsigned long long)r * p.C) & Lm);
+        // G_j = (-n*(8b + j)) mod L, stepped by -(n mod L) (DigitWalk)
+        unsigned Gj = (unsigned)((0ull - p.n * (unsigned long long)(b * 8)) & Lm);
         double q1 = 0, q2 = 0, q3 = 0;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             double2 wv = wri[u];
-            unsigned f = F + (unsigned)((0ull - p.n * (unsigned long long)(b * 8 + j)) & Lm);
+            unsigned f = F + Gj;
+            Gj = Gj >= nr ? Gj - nr : Gj + p.L - nr;
             if (f >= p.L) {
                 f -= p.L;
                 wv = make_double2(2.0 * wv.x, 2.0 * wv.y);
@@ -305,11 +308,12 @@ __device__ __forceinline__ bool loop_outer(const Params& p, double2* const (&buf
         long long nu[8];
         ripple<8>(p, r * p.C + b * 8, c2in, extra, nu);
         const unsigned F = (unsigned)((0ull - p.n * (unsigned long long)r * p.C) & Lm);
+        unsigned Gj = (unsigned)((0ull - p.n * (unsigned long long)(b * 8)) & Lm);
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             double2 wv = wrf[u];
-            if (F + (unsigned)((0ull - p.n * (unsigned long long)(b * 8 + j)) & Lm) >= p.L)
-                wv = make_double2(0.5 * wv.x, 0.5 * wv.y);
+            if (F + Gj >= p.L) wv = make_double2(0.5 * wv.x, 0.5 * wv.y);
+            Gj = Gj >= nr ? Gj - nr : Gj + p.L - nr;
             const double xd = (double)(dg[u][j] + (int)nu[j]);
             io[Lay::at(sub * 8 + j, r)] = make_double2(xd * wv.x, xd * wv.y);
         }
