@@ -217,3 +217,27 @@ def test_square_long_lengths_vs_powm(gpu, k, n, L, iters):
         print(f"[parity] squared, max round-off {e:.4f}; GMP check", flush=True)
         assert 0 < e < 0.3
         assert c.res64() == G.powm(x, 1 << iters, N) & M64
+
+
+@pytest.mark.parametrize("k,n", [(433917, 2097152), (11887, 560001)])
+def test_resident_chunks_and_their_event_times(gpu, monkeypatch, k, n):
+    """Whole and partial chunks of the resident loop in one call (the chunks
+    are pipelined: each one is enqueued before the one ahead of it is read
+    back), against GMP; pl_loop_time counts one resident launch per chunk
+    of at least two squarings, carrying all but each chunk's first and last
+    squaring, and its event time is positive."""
+    monkeypatch.setenv("PL_RESIDENT", "1")
+    N = k * (1 << n) + 1
+    x = random.Random(n).randrange(N)
+    iters = 256 * 2 + 77
+    print(f"[parity] resident chunks k={k} n={n} iters={iters}", flush=True)
+    with gpu.Context(k, n) as c:
+        assert c.info.passes == 0
+        c.set_residue(x)
+        c.loop_time(reset=True)
+        e, tr = c.square(iters, trace=True)
+        lt = c.loop_time(reset=True)
+        assert c.get_residue() == G.powm(x, 1 << iters, N)
+        assert 0 < e < 0.3 and len(tr) == iters and all(0 <= t < 0.3 for t in tr)
+        assert lt["launches"] == 3 and lt["squarings"] == 255 * 2 + 76 and lt["ms"] > 0
+        assert c.loop_time()["launches"] == 0
