@@ -238,7 +238,6 @@ class Group {
     int* rdig_ = nullptr;
     long long* xhi_ = nullptr;
     unsigned long long* xw_ = nullptr;  // per slot: X_hi as a tagged word (k_loop's row passes poll it)
-    unsigned* lflags_ = nullptr;        // per slot: k_loop's flag words (256 column tiles, 256 row tiles)
     unsigned long long* err_ = nullptr;   // cap x kChunk (bit patterns of non-negative doubles)
     unsigned long long* errp_ = nullptr;  // cap x eb_ x kChunk per-block partials
     uint32_t eb_ = 1;                     // blocks that report round-off per squaring

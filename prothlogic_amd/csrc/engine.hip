@@ -111,7 +111,6 @@ struct Params {
                             // entering a k_loop launch)
     unsigned poll_sleep;    // k_loop: pause between reloads of points not yet ready
                             // (units of s_sleep 1, about 64 clocks)
-    unsigned* lflags;       // k_loop with flag words: per slot 256 column-tile + 256 row-tile words
     // bound per slot by bind()
     unsigned long long n;
     long long k;
@@ -142,7 +141,6 @@ __device__ __forceinline__ Params bind(const Params& base, unsigned y)
     p.dig += (size_t)slot * base.L;
     p.runc += (size_t)slot * base.runc_stride;
     p.epoch += 4 * (size_t)slot;
-    p.lflags += 512 * (size_t)slot;
     p.carries += (size_t)slot * base.carr_stride;
     p.xhi += slot;
     p.err += (size_t)slot * base.err_stride;  // err_stride = blocks * kChunk
@@ -1463,24 +1461,21 @@ static void dispatch_row_mul(const Params& p, const double2* zo, const long long
 // digits per thread (threads past the tile's runs wait).
 struct LoopShape {
     uint32_t R, C, ntd, ntr, tpb, cr, rr;
-    bool flags;  // flag words per pass besides the tagged points
     const void* kern;
 };
-#define PL_LOOP(R_, C_, NTD_, NTR_, T_, CR_, RR_, F_) \
-    { R_, C_, NTD_, NTR_, T_, CR_, RR_, F_, reinterpret_cast<const void*>(&k_loop<R_, C_, NTD_, NTR_, T_, CR_, RR_, F_>) }
+#define PL_LOOP(R_, C_, NTD_, NTR_, T_, CR_, RR_) \
+    { R_, C_, NTD_, NTR_, T_, CR_, RR_, reinterpret_cast<const void*>(&k_loop<R_, C_, NTD_, NTR_, T_, CR_, RR_>) }
 static const LoopShape kLoopShapes[] = {
     // preferred variant of each shape first (measured on MI355X)
-    PL_LOOP(64, 512, 8, 1, 128, 4, 4, false),      // L = 2^15: 64 workgroups
-    PL_LOOP(128, 512, 8, 2, 256, 4, 4, false),     // L = 2^16: 64 workgroups
-    PL_LOOP(128, 1024, 8, 1, 256, 4, 4, false),    // L = 2^17: 128 workgroups
-    PL_LOOP(128, 1024, 8, 1, 256, 4, 4, true),
-    PL_LOOP(128, 1024, 8, 1, 128, 8, 8, false),
-    PL_LOOP(256, 512, 8, 4, 256, 8, 8, false),     // L = 2^17: 64 workgroups
-    PL_LOOP(128, 2048, 16, 1, 512, 4, 4, false),   // L = 2^18: 128 workgroups
-    PL_LOOP(256, 1024, 8, 2, 256, 8, 8, false),    // L = 2^18: 128 workgroups
-    PL_LOOP(256, 2048, 16, 2, 512, 8, 8, false),   // L = 2^19: 128 workgroups
-    PL_LOOP(256, 4096, 16, 1, 512, 8, 8, false),   // L = 2^20: 256 workgroups
-    PL_LOOP(256, 4096, 16, 1, 512, 8, 8, true),
+    PL_LOOP(64, 512, 8, 1, 128, 4, 4),      // L = 2^15: 64 workgroups
+    PL_LOOP(128, 512, 8, 2, 256, 4, 4),     // L = 2^16: 64 workgroups
+    PL_LOOP(128, 1024, 8, 1, 256, 4, 4),    // L = 2^17: 128 workgroups
+    PL_LOOP(128, 1024, 8, 1, 128, 8, 8),
+    PL_LOOP(256, 512, 8, 4, 256, 8, 8),     // L = 2^17: 64 workgroups
+    PL_LOOP(128, 2048, 16, 1, 512, 4, 4),   // L = 2^18: 128 workgroups
+    PL_LOOP(256, 1024, 8, 2, 256, 8, 8),    // L = 2^18: 128 workgroups
+    PL_LOOP(256, 2048, 16, 2, 512, 8, 8),   // L = 2^19: 128 workgroups
+    PL_LOOP(256, 4096, 16, 1, 512, 8, 8),   // L = 2^20: 256 workgroups
 };
 #undef PL_LOOP
 constexpr int kNumLoopShapes = (int)(sizeof(kLoopShapes) / sizeof(kLoopShapes[0]));
@@ -1660,7 +1655,7 @@ size_t Group::slot_bytes(const Plan& p)
 
 // Byte offsets of a group's buffers in its one device allocation.
 struct GroupLayout {
-    size_t z, car, runc, runt, epoch, rdig, xhi, xw, lflags, err, errp, st, dig, desc, state, zero_end;
+    size_t z, car, runc, runt, epoch, rdig, xhi, xw, err, errp, st, dig, desc, state, zero_end;
     size_t tw, tlo, thi, olo, ohi, plo, phi, ilo, ihi, tcq, wrf, wri, wcf, wci, bytes;
     uint32_t carry_tpb, carry_blocks, eb;
 };
@@ -1687,7 +1682,6 @@ static GroupLayout group_layout(const Plan& p, uint32_t cap)
     put(g.rdig, p.nt ? sizeof(int) * (size_t)p.L * cap : 0);
     put(g.xhi, sizeof(long long) * cap);
     put(g.xw, sizeof(unsigned long long) * cap);
-    put(g.lflags, sizeof(unsigned) * 512 * cap);
     put(g.err, sizeof(unsigned long long) * Group::kChunk * cap);
     put(g.errp, sizeof(unsigned long long) * Group::kChunk * g.eb * cap);
     put(g.st, sizeof(int));
@@ -1779,7 +1773,6 @@ void Group::init(void* ext_mem, size_t ext_bytes, hipStream_t ext_stream)
     rdig_ = reinterpret_cast<int*>(b + o_rdig);
     xhi_ = reinterpret_cast<long long*>(b + o_xhi);
     xw_ = reinterpret_cast<unsigned long long*>(b + lay.xw);
-    lflags_ = reinterpret_cast<unsigned*>(b + lay.lflags);
     err_ = reinterpret_cast<unsigned long long*>(b + o_err);
     errp_ = reinterpret_cast<unsigned long long*>(b + o_errp);
     status_ = reinterpret_cast<int*>(b + o_st);
@@ -1905,7 +1898,6 @@ Params Group::params(uint32_t slot0, bool check_active) const
     p.per_slot = uniform_ ? 0 : 1;
     p.ztag = 0;
     p.poll_sleep = poll_sleep_;
-    p.lflags = lflags_;
     p.n = plan_.n;
     p.k = (long long)plan_.k;
     p.mult = 1;

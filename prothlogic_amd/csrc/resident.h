@@ -164,53 +164,16 @@ __device__ __forceinline__ bool poll_word(const unsigned long long* w, unsigned 
     return true;
 }
 
-// Flag handshake (variants with FLAGS): besides the tagged points, each
-// workgroup publishes one word per pass, the epoch tag of the squaring whose
-// pass it has finished, after all its stores have completed; a consumer waits
-// for every producer's word before it loads (one wave polls, the others wait
-// at the barrier).  The points then arrive ready on their first load.
-__device__ __forceinline__ bool wait_flags(const unsigned* f, unsigned G, unsigned tag, const Params& p)
-{
-    bool ok = true;
-    if (threadIdx.x < 64) {
-        unsigned spins = 0;
-        for (;;) {
-            bool ready = true;
-            for (unsigned v = threadIdx.x; v < G; v += 64)
-                ready = ready && (__hip_atomic_load(f + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kTagMask) == tag;
-            if (__all(ready)) break;
-            if (++spins >= p.spin_limit || ((spins & 63) == 0 && launch_aborted(p))) {
-                ok = false;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        if (!ok && threadIdx.x == 0) atomicOr(p.status, 2);
-    }
-    return !__syncthreads_or(!ok);
-}
-
-__device__ __forceinline__ void publish_flag(unsigned* f, unsigned w, unsigned tag)
-{
-    __builtin_amdgcn_s_waitcnt(0);  // this thread's device-scope stores have completed
-    __syncthreads();                // and every other thread's
-    if (threadIdx.x == 0) __hip_atomic_store(f + w, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Outer pass of one squaring over the column tile of workgroup w: columns
 // [w*NTD, w*NTD + NTD), NTD/8 runs of 8 digits per row.  Thread q handles
 // runs q + u*T (u < RPT) of the tile: row r = run % R, run column block
 // w*(NTD/8) + run / R.  Otherwise as k_carry_turn, with its point loads and
 // stores replaced by tagged device-scope ones and X_hi published as a
 // tagged word.
-template <int R, int NTD, int T, int MAXR, bool FLAGS>
+template <int R, int NTD, int T, int MAXR>
 __device__ __forceinline__ bool loop_outer(const Params& p, double2* const (&buf)[2], const PtBuf& zb, unsigned w,
-                                           unsigned e, unsigned ecol, double* s_err, unsigned long long* xw,
-                                           bool first)
+                                           unsigned e, unsigned ecol, double* s_err, unsigned long long* xw)
 {
-    // row tiles' flags: the previous squaring's row passes (the launch's
-    // first outer pass reads what the kernel before it wrote)
-    if (FLAGS && !first && !wait_flags(p.lflags + 256, p.C / NTD, (e + kTagMask) & kTagMask, p)) return false;
     using Lay = LdsLayout<R, NTD>;
     constexpr int BPT = NTD / 8;           // run column blocks per tile
     constexpr int RUNS = R * BPT;
@@ -369,17 +332,15 @@ __device__ __forceinline__ bool loop_outer(const Params& p, double2* const (&buf
             zb.store((unsigned)idx * p.C + c0 + tr, cmul(v, ct.w[u][r]), 0u);
         },
         tid);
-    if (FLAGS) publish_flag(p.lflags, w, e);
     return true;
 }
 
 // Row pass of one squaring over rows [w*NTR, w*NTR + NTR), as k_row, with
 // tagged device-scope point loads and stores and X_hi from its tagged word.
-template <int C, int NTR, int T, int RADIX, bool FLAGS>
+template <int C, int NTR, int T, int RADIX>
 __device__ __forceinline__ bool loop_row(const Params& p, double2* const (&buf)[2], const PtBuf& zb, unsigned w,
                                          unsigned e, const unsigned long long* xw)
 {
-    if (FLAGS && !wait_flags(p.lflags, p.R / NTR, e, p)) return false;
     constexpr int P0 = Radix<C, RADIX>::first();
     using S = StageShape<C, NTR, T, P0>;
     const unsigned tid = pass_tid();
@@ -400,7 +361,6 @@ __device__ __forceinline__ bool loop_row(const Params& p, double2* const (&buf)[
         [&](double2 X) { return cmul(X, make_double2(X.x, X.y + x2)); },
         [&](int tr, int idx, double2 v, int, int) { zb.store(base + (unsigned)tr * C + (unsigned)idx, v, 1u); },
         tid);
-    if (FLAGS) publish_flag(p.lflags + 256, w, e);
     return true;
 }
 
@@ -409,11 +369,10 @@ __device__ __forceinline__ bool loop_row(const Params& p, double2* const (&buf)[
 // after a row pass that stored parity 1; left after a row pass (parity 1),
 // so the exit kernels follow as after k_row.  Epoch and round-off column
 // advance as nsq k_carry_turn + k_row pairs would advance them.
-template <int R, int C, int NTD, int NTR, int T, int MAXR, int RRADIX, bool FLAGS>
+template <int R, int C, int NTD, int NTR, int T, int MAXR, int RRADIX>
 __global__ __launch_bounds__(T) void k_loop(Params pb, unsigned nsq, unsigned long long* xw)
 {
     static_assert(C / NTD == R / NTR, "one column tile and one row tile per workgroup");
-    static_assert(C / NTD <= 256, "flag words per pass");
     constexpr int SC = LdsLayout<R, NTD>::kSlots, SR = LdsLayout<C, NTR>::kSlots;
     constexpr int SL = SC > SR ? SC : SR;
     __shared__ double2 lds[2][SL];
@@ -435,10 +394,9 @@ __global__ __launch_bounds__(T) void k_loop(Params pb, unsigned nsq, unsigned lo
         q.wrf = opaque(q.wrf);
         q.wci = opaque(q.wci);
         q.wcf = opaque(q.wcf);
-        if (!loop_outer<R, NTD, T, MAXR, FLAGS>(q, buf, zb, w, e, (col0 + s) & (Group::kChunk - 1), s_err, xw, s == 0))
-            break;
+        if (!loop_outer<R, NTD, T, MAXR>(q, buf, zb, w, e, (col0 + s) & (Group::kChunk - 1), s_err, xw)) break;
         q.tw = opaque(q.tw);
-        if (!loop_row<C, NTR, T, RRADIX, FLAGS>(q, buf, zb, w, e, xw)) break;
+        if (!loop_row<C, NTR, T, RRADIX>(q, buf, zb, w, e, xw)) break;
     }
     // every workgroup read the epochs before block 0 can get here: block 0's
     // last row pass waited for every tile's last outer pass
