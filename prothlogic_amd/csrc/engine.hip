@@ -111,6 +111,8 @@ struct Params {
                             // entering a k_loop launch)
     unsigned poll_sleep;    // k_loop: pause between reloads of points not yet ready
                             // (units of s_sleep 1, about 64 clocks)
+    unsigned step_off;      // scheduled stepping: steps of this chunk before this launch's
+                            // (the slot states advance once per chunk, k_advance)
     // bound per slot by bind()
     unsigned long long n;
     long long k;
@@ -155,9 +157,14 @@ __device__ __forceinline__ Params bind(const Params& base, unsigned y)
         p.n = d.n;
         p.k = d.k;
         if (base.sched) {
+            // the state as of the chunk's start, moved on by the chunk's
+            // steps before this one (a slot stops once it has carried
+            // total - 1 squarings; at step_off 0 the state is taken as it is,
+            // which lets the leave sequence run a finished slot)
             const SlotState st = base.state[slot];
-            p.active = st.active;
-            const unsigned long long i = st.carried + 1;  // squaring being carried
+            const unsigned long long c = st.carried + base.step_off;
+            p.active = st.active && (base.step_off == 0 || c + 1 < d.total);
+            const unsigned long long i = c + 1;  // squaring being carried
             if (d.a && i <= d.topbit && ((d.k >> (d.topbit - i)) & 1)) p.mult = d.a;
         }
     }
@@ -1169,15 +1176,21 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ 
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
 }
 
-// End of a scheduled step: a slot that has carried total-1 squarings stops
-// (its last squaring is carried by the leave sequence).
-__global__ void k_advance(SlotState* st, const SlotDesc* desc, unsigned slots)
+// End of a chunk of scheduled steps: a slot that has carried total-1
+// squarings stops (its last squaring is carried by the leave sequence).
+__global__ void k_advance(SlotState* st, const SlotDesc* desc, unsigned slots, unsigned g)
 {
+    // g steps at once: the slot stops at the first step after which it has
+    // carried total - 1 squarings
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots || !st[i].active) return;
-    const unsigned long long c = st[i].carried + 1;
-    st[i].carried = c;
-    if (c + 1 >= desc[i].total) st[i].active = 0;
+    const unsigned long long c0 = st[i].carried, total = desc[i].total;
+    if (c0 + 1 + g >= total) {
+        st[i].carried = total - 1;
+        st[i].active = 0;
+    } else {
+        st[i].carried = c0 + g;
+    }
 }
 
 // ------------------------------------------------------------- host side
@@ -1898,6 +1911,7 @@ Params Group::params(uint32_t slot0, bool check_active) const
     p.per_slot = uniform_ ? 0 : 1;
     p.ztag = 0;
     p.poll_sleep = poll_sleep_;
+    p.step_off = 0;
     p.n = plan_.n;
     p.k = (long long)plan_.k;
     p.mult = 1;
@@ -1927,7 +1941,8 @@ void Group::launch(int kind, const Params& p, uint32_t slots)
         break;
     }
     case kAdvance:
-        hipLaunchKernelGGL(k_advance, dim3((slots + 63) / 64), dim3(64), 0, stream_, state_, desc_, slots);
+        hipLaunchKernelGGL(k_advance, dim3((slots + 63) / 64), dim3(64), 0, stream_, state_, desc_, slots,
+                           p.step_off);
         break;
     default: dispatch_col(kind, p, slots, plan_.nt, plan_.col_radix, stream_);
     }
@@ -2077,7 +2092,8 @@ void Group::enqueue_exit(const Params& p, uint32_t slots)
 void Group::enqueue_step_chunk(int g)
 {
     for (int i = 0; i < g; ++i) {
-        const Params p = params(0, true);
+        Params p = params(0, true);
+        p.step_off = (unsigned)i;
         if (plan_.nt) {
             launch(kInvCarry, p, step_n_);
             launch(kAdjFwd, p, step_n_);
@@ -2088,8 +2104,11 @@ void Group::enqueue_step_chunk(int g)
             launch(kFwdCol, p, step_n_);
         }
         launch(kRow, p, step_n_);
-        launch(kAdvance, p, step_n_);
     }
+    // the slots' states move on once for the whole chunk
+    Params pa = params(0, true);
+    pa.step_off = (unsigned)g;
+    launch(kAdvance, pa, step_n_);
 }
 
 void Group::build_graph(hipGraphExec_t& exec, hipGraph_t& graph, bool stepping, int g, bool use_turn)
