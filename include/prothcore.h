@@ -37,7 +37,7 @@ extern "C" {
 #define PL_ERR_STATE (-4)    /* device carry did not settle (internal bug)  */
 #define PL_ERR_IO (-5)       /* checkpoint file could not be read/written   */
 
-#define PL_ABI_VERSION 5
+#define PL_ABI_VERSION 6
 
 typedef struct pl_ctx pl_ctx;
 
