@@ -21,6 +21,9 @@ host's cores for bounded samples).
 
   python bench.py [--gpus 1] [--steps 10000] [--warmup 512] [--no-cpu] [--no-tune]
   python bench.py --c4-full OUT.json     config 4 at its stated size
+  python bench.py --c5-full S NS RATE OUT.json    segment S of NS of config 5 at its stated size
+  python bench.py --plain-keys OUT.json  the plain loop's time per (length, carry run) key of configs 4/5
+  python bench.py --c5-combine PLAIN.json GMP.jsonl SEG.json...   summary of config-5 segments (CPU only)
   python bench.py --cpu-only             the CPU legs alone
 """
 from __future__ import annotations
@@ -594,6 +597,131 @@ def serve_c5(core, workloads, count=200, scale=100, load=2.0, seed=5):
                 makespan_s=round(makespan, 3), tuned_plans=tuned)
 
 
+def plain_keys(core, workloads, out_path):
+    """The plain one-at-a-time loop's steady time per squaring for every
+    (transform length, carry run) key that the config-4 and config-5
+    candidates form (a single context's speed depends on both: runs of 16
+    digits take the split outer pass).  Each candidate's key is read from a
+    context; each key's first candidate is tuned and timed alone, best of two
+    1536-squaring runs.  Writes {"keys": [key of each config-5 candidate, by
+    its index, 843*2^120002+1 last], "keys_c4": [the same for config 4,
+    3*2^303093+1 last], "plain": {key: {k, n, s, plan}}}; keys read "L/run"."""
+    c5 = list(workloads.c5().candidates) + [(843, 120002)]
+    c4 = list(workloads.c4().candidates) + [(3, 303093)]
+    keys = []
+    for k, n in c5 + c4:
+        c = core.Context(k, n)
+        keys.append(f"{c.fft_len}/{c.info.run}")
+        c.close()
+    first = {}
+    for i, key in enumerate(keys):
+        first.setdefault(key, i)
+    plain = {}
+    for key, i in sorted(first.items(), key=lambda x: tuple(map(int, x[0].split("/")))):
+        k, n = (c5 + c4)[i]
+        r = core.autotune_plan(k, n) if TUNE else None
+        s = min(plain_per_squaring(core, k, n, iters=1536)[0] for _ in range(2))
+        plain[key] = dict(k=k, n=n, s=s, plan=_plan_str(r["best"]) if r else "rule")
+        log(f"[bench] plain {key}: {s * 1e6:.2f} us per squaring ({plain[key]['plan']})")
+    d = dict(keys=keys[:len(c5)], keys_c4=keys[len(c5):], plain=plain)
+    Path(out_path).write_text(json.dumps(d))
+    return {key: round(v["s"] * 1e6, 2) for key, v in plain.items()}
+
+
+def c5_combine(plain_path, gmp_path, seg_paths):
+    """Text summary of full-length config-5 segments (`--c5-full` outputs):
+    time to verdict, verdicts, throughput, and the plain one-at-a-time loop
+    replayed on the same arrivals (FIFO, one test at a time, each taking its
+    squarings times the steady per-squaring time of its key from
+    `--plain-keys`), plus the GMP check lines of tests/gmp_check_results.py
+    when gmp_path names a file ("-": none)."""
+    import numpy as np
+    data = sorted((json.loads(Path(p).read_text()) for p in seg_paths), key=lambda d: d["segment"])
+    keyed = json.loads(Path(plain_path).read_text())
+    nseg, rate = data[0]["segments"], data[0]["rate"]
+    rows = [r for d in data for r in d["tests"]]
+    lat = np.array([r["latency_s"] for r in rows])
+
+    def squarings(r):
+        return r["n"] - 1 + r["k"].bit_length() - 1
+
+    def plain_s(r):
+        return keyed["plain"][keyed["keys"][r["tag"]]]["s"]
+
+    plat = []
+    for d in data:
+        free = 0.0
+        for r in d["tests"]:
+            free = max(r["arrival_s"], free) + plain_s(r) * squarings(r)
+            plat.append(free - r["arrival_s"])
+    plat = np.array(plat)
+    mk = sum(d["makespan_s"] for d in data)
+    busy = sum(plain_s(r) * squarings(r) for r in rows)
+    tph, ptph = 3600 * len(rows) / mk, 3600 * len(rows) / busy
+    ver = dict(prime=sum(r["verdict"] == 1 for r in rows), composite=sum(r["verdict"] == 0 for r in rows),
+               void=sum(r["error"] != 0 for r in rows))
+    kp = [r for r in rows if (r["k"], r["n"]) == (843, 120002)]
+    by_key = ", ".join(f"L={key.replace('/', ' run')}: {v['s'] * 1e6:.2f}" for key, v in
+                       sorted(keyed["plain"].items(), key=lambda x: tuple(map(int, x[0].split("/")))))
+    out = [
+        "# Config 5 at its stated size: 2000 Poisson arrivals, every test run to its verdict (serving)",
+        "",
+        "commands (from the repository root; the first two on the GPU host, one bounded session per segment):",
+        "",
+        f"    python3 bench.py --no-cpu --c5-full S {nseg} {rate} gpurun_out/c5seg/segS_of_{nseg}.json     "
+        f"(S = {', '.join(str(d['segment']) for d in data)})",
+        "    python3 bench.py --no-cpu --plain-keys gpurun_out/plain_keys.json",
+        "    python3 tests/gmp_check_results.py --out gpurun_out/c5_gmp.jsonl gpurun_out/c5seg/seg*_of_8.json",
+        "    python3 bench.py --c5-combine gpurun_out/plain_keys.json gpurun_out/c5_gmp.jsonl "
+        "gpurun_out/c5seg/seg*_of_8.json",
+        "",
+        "Workload: the 2000 config-5 candidates (k odd in [3,2^20), n log-uniform in [5e4,4e6], PCG64 seed 5, gate",
+        f"survivors) arriving as one Poisson process at {rate} tests/s (exponential gaps, PCG64 seed 5), plus the",
+        f"known prime 843*2^120002+1 right after the first arrival.  The arrival sequence is cut into {nseg} "
+        "consecutive",
+        "segments of 250 arrivals; each segment replays its arrivals (times shifted to start at 0) into a fresh "
+        "pool in",
+        f"its own GPU session.  Segments run: {[d['segment'] for d in data]} of 0..{nseg - 1} ({len(rows)} of the "
+        "2001 tests).",
+        "Time to verdict = verdict time - submission time of each test.  The plain comparison replays the same",
+        "arrivals through one test at a time (FIFO, one context), each test taking its squarings times the",
+        "steady per-squaring time of a single context of its (length, carry run) key (--plain-keys).",
+        "",
+        f"tests                      {len(rows)}",
+        f"verdicts                   {ver}",
+        f"known prime                843*2^120002+1: "
+        + (f"verdict {kp[0]['verdict']}, Res64 {kp[0]['res64']}" if kp else "not in these segments"),
+        f"max round-off              {max(r['max_err'] for r in rows)}",
+        f"pool p50 / p99 / max       {np.percentile(lat, 50):.2f} / {np.percentile(lat, 99):.2f} / "
+        f"{lat.max():.2f} s time to verdict",
+        f"plain p50 / p99 / max      {np.percentile(plat, 50):.1f} / {np.percentile(plat, 99):.1f} / "
+        f"{plat.max():.1f} s",
+        f"makespan (sum of segments) {mk:.1f} s  -> {tph:.1f} tests/h completed (the pool, not the arrivals, "
+        f"bounds it:",
+        f"                           it serves this mix at about {len(rows) / mk:.2f} tests/s against "
+        f"{rate} arriving)",
+        f"plain loop busy time       {busy:.1f} s  -> its capacity {ptph:.1f} tests/h",
+        f"pool / plain throughput    {tph / ptph:.2f}x",
+        f"plain us/squaring by key   {by_key}",
+        "",
+    ]
+    if gmp_path != "-" and Path(gmp_path).exists():
+        got = {(r["k"], r["n"]): r for r in rows}
+        gmp = [g for g in map(json.loads, Path(gmp_path).read_text().splitlines()) if (g["k"], g["n"]) in got]
+        if gmp:
+            ok = sum(g["match"] and g["device"] == got[(g["k"], g["n"])]["res64"] for g in gmp)
+            out += ["GMP check (tests/gmp_check_results.py: mpz_powm, the reference's base rule; every test of these",
+                    f"segments with n < 120000): {len(gmp)} tests, n {min(g['n'] for g in gmp)}.."
+                    f"{max(g['n'] for g in gmp)}, k {min(g['k'] for g in gmp)}..{max(g['k'] for g in gmp)}; "
+                    f"Res64 and verdict equal GMP's: {ok} of {len(gmp)}", ""]
+    out.append("per segment:")
+    for d in data:
+        sl = np.array([r["latency_s"] for r in d["tests"]])
+        out.append(f"    segment {d['segment']}: {len(d['tests'])} tests, makespan {d['makespan_s']:.1f} s, "
+                   f"p50 {np.percentile(sl, 50):.2f} s, p99 {np.percentile(sl, 99):.2f} s")
+    return "\n".join(out) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -608,6 +736,12 @@ def main():
     ap.add_argument("--c5-full", nargs=4, metavar=("SEG", "NSEG", "RATE", "OUT.json"), default=None,
                     help="only run segment SEG of NSEG of config 5 at its stated size (Poisson arrivals at RATE "
                          "tests/s, every test to its verdict); per-test results to OUT")
+    ap.add_argument("--plain-keys", metavar="OUT.json", default=None,
+                    help="only time the plain loop per (transform length, carry run) key of the config-4 and "
+                         "config-5 candidates; to OUT")
+    ap.add_argument("--c5-combine", nargs="+", metavar="FILE", default=None,
+                    help="PLAIN.json GMP.jsonl|- SEG.json...: print the summary of full-length config-5 segments "
+                         "(no GPU needed)")
     ap.add_argument("--cpu-only", action="store_true", help="only the CPU legs (no GPU needed)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-plan autotuner (plans from --plans, else the built-in rule); for "
@@ -619,6 +753,11 @@ def main():
     if args.cpu_sample:
         kind, k, n, sec = args.cpu_sample
         print(json.dumps(cpu_sample(kind, int(k), int(n), float(sec))), flush=True)
+        return
+    if args.c5_combine:
+        if len(args.c5_combine) < 3:
+            ap.error("--c5-combine needs PLAIN.json, GMP.jsonl (or -) and at least one segment file")
+        sys.stdout.write(c5_combine(args.c5_combine[0], args.c5_combine[1], args.c5_combine[2:]))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every configuration fits one MI355X and candidates are "
@@ -650,6 +789,9 @@ def main():
         core.tune_load(args.plans)
     if args.c4_full:
         print(json.dumps(batch_c4_full(core, workloads, args.c4_full)), flush=True)
+        return
+    if args.plain_keys:
+        print(json.dumps(plain_keys(core, workloads, args.plain_keys)), flush=True)
         return
     if args.c5_full:
         seg, nseg, rate, outp = args.c5_full
