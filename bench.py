@@ -710,10 +710,14 @@ def c5_combine(plain_path, gmp_path, seg_paths):
         gmp = [g for g in map(json.loads, Path(gmp_path).read_text().splitlines()) if (g["k"], g["n"]) in got]
         if gmp:
             ok = sum(g["match"] and g["device"] == got[(g["k"], g["n"])]["res64"] for g in gmp)
-            out += ["GMP check (tests/gmp_check_results.py: mpz_powm, the reference's base rule; every test of these",
-                    f"segments with n < 120000): {len(gmp)} tests, n {min(g['n'] for g in gmp)}.."
-                    f"{max(g['n'] for g in gmp)}, k {min(g['k'] for g in gmp)}..{max(g['k'] for g in gmp)}; "
-                    f"Res64 and verdict equal GMP's: {ok} of {len(gmp)}", ""]
+            top = max(g["n"] for g in gmp)
+            checked = {(g["k"], g["n"]) for g in gmp}
+            below = [r for r in rows if r["n"] <= top]
+            out += ["GMP check (tests/gmp_check_results.py: mpz_powm, the reference's base rule):",
+                    f"    {len(gmp)} tests, n {min(g['n'] for g in gmp)}..{top}, k {min(g['k'] for g in gmp)}.."
+                    f"{max(g['k'] for g in gmp)}: {sum((r['k'], r['n']) in checked for r in below)} of the "
+                    f"{len(below)} tests with n <= {top}",
+                    f"    Res64 and verdict equal GMP's: {ok} of {len(gmp)}", ""]
     out.append("per segment:")
     for d in data:
         sl = np.array([r["latency_s"] for r in d["tests"]])
