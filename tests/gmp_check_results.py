@@ -71,11 +71,15 @@ def main() -> int:
     print(f"{len(todo)} tests to check", file=sys.stderr, flush=True)
     out = open(args.out, "a") if args.out else sys.stdout
     bad = 0
-    with ProcessPoolExecutor(args.procs) as ex:
-        for o in ex.map(check, todo):
-            bad += not o["match"]
-            out.write(json.dumps(o) + "\n")
-            out.flush()
+    try:
+        with ProcessPoolExecutor(args.procs) as ex:
+            for o in ex.map(check, todo):
+                bad += not o["match"]
+                out.write(json.dumps(o) + "\n")
+                out.flush()
+    finally:
+        if out is not sys.stdout:
+            out.close()
     return 1 if bad else 0
 
 
