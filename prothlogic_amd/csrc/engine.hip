@@ -1479,14 +1479,15 @@ struct LoopShape {
 #define PL_LOOP(R_, C_, NTD_, NTR_, T_, CR_, RR_) \
     { R_, C_, NTD_, NTR_, T_, CR_, RR_, reinterpret_cast<const void*>(&k_loop<R_, C_, NTD_, NTR_, T_, CR_, RR_>) }
 static const LoopShape kLoopShapes[] = {
-    // preferred variant of each shape first (measured on MI355X)
+    // preferred variant of each shape first, and for each length the shape an
+    // untuned context takes first (measured on MI355X)
     PL_LOOP(64, 512, 8, 1, 128, 4, 4),      // L = 2^15: 64 workgroups
     PL_LOOP(128, 512, 8, 2, 256, 4, 4),     // L = 2^16: 64 workgroups
     PL_LOOP(128, 1024, 8, 1, 256, 4, 4),    // L = 2^17: 128 workgroups
     PL_LOOP(128, 1024, 8, 1, 128, 8, 8),
     PL_LOOP(256, 512, 8, 4, 256, 8, 8),     // L = 2^17: 64 workgroups
-    PL_LOOP(128, 2048, 16, 1, 512, 4, 4),   // L = 2^18: 128 workgroups
-    PL_LOOP(256, 1024, 8, 2, 256, 8, 8),    // L = 2^18: 128 workgroups
+    PL_LOOP(256, 1024, 8, 2, 256, 8, 8),    // L = 2^18: 128 workgroups (13.5 us per squaring)
+    PL_LOOP(128, 2048, 16, 1, 512, 4, 4),   // L = 2^18: 128 workgroups (14.9 us)
     PL_LOOP(256, 2048, 16, 2, 512, 8, 8),   // L = 2^19: 128 workgroups
     PL_LOOP(256, 4096, 16, 1, 512, 8, 8),   // L = 2^20: 256 workgroups
 };

@@ -52,7 +52,7 @@ def test_square_exact_vs_python_int(gpu, k, n, L):
 # (PL_CARRY_TURN=0: two launches; 3 kernels), "unfused" (PL_FUSED=0: 5).
 @pytest.mark.parametrize("mode", ["resident", "turn", "split", "unfused"])
 @pytest.mark.parametrize("k,n,L", [(3, 20000, 2048), (1048573, 60000, 4096), (11887, 300007, 16384),
-                                   (433917, 2097152, 131072), (65535, 3000017, 262144)])
+                                   (433917, 2097152, 131072), (65535, 3000017, 262144), (3, 4000037, 262144)])
 def test_fused_and_unfused_paths(gpu, monkeypatch, mode, k, n, L):
     monkeypatch.setenv("PL_FUSED", "0" if mode == "unfused" else "1")
     monkeypatch.setenv("PL_CARRY_TURN", "0" if mode == "split" else "1")
