@@ -671,7 +671,8 @@ def c5_combine(plain_path, gmp_path, seg_paths):
         f"    python3 bench.py --no-cpu --c5-full S {nseg} {rate} gpurun_out/c5seg/segS_of_{nseg}.json     "
         f"(S = {', '.join(str(d['segment']) for d in data)})",
         "    python3 bench.py --no-cpu --plain-keys gpurun_out/plain_keys.json",
-        "    python3 tests/gmp_check_results.py --out gpurun_out/c5_gmp.jsonl gpurun_out/c5seg/seg*_of_8.json",
+        "    python3 tests/gmp_check_results.py --nmax N --out gpurun_out/c5_gmp.jsonl gpurun_out/c5seg/seg*_of_8.json"
+        "     (in the container; N below)",
         "    python3 bench.py --c5-combine gpurun_out/plain_keys.json gpurun_out/c5_gmp.jsonl "
         "gpurun_out/c5seg/seg*_of_8.json",
         "",
